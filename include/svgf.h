@@ -1,0 +1,231 @@
+/*
+ * svgf.h — C ABI of the MI355X-native SVGF denoiser (gfx950, HIP).
+ *
+ * Drop-in replacement for the SVGF compute pass of TronGrid:
+ *   - the shader          src/svgf_denoise.slang:61-275 (bindings, push block, svgfMain)
+ *   - its host recording  src/main.cpp:907-947 (4 dispatches + barriers inside recordFrame)
+ *   - its resources       src/main.cpp:1326-1496 (7 images, zero-init = cold history)
+ *   - its descriptor sets src/main.cpp:2288-2446 (moments / denoised ping-pong)
+ *
+ * Conventions
+ *   - Every function returns an svgf_status; nothing throws or aborts across the ABI.
+ *   - Planes are plain device pointers with a row pitch in bytes. The caller owns every
+ *     per-frame plane; the library owns only its scratch and (batched form) per-stream
+ *     history, exactly as TronGrid's render thread owns its images (main.cpp:1333-1370).
+ *   - `hip_stream` is a hipStream_t passed as void* (NULL = the default stream). All work is
+ *     enqueued on it; calls return before the GPU finishes, like vkCmdDispatch.
+ *   - A context is externally synchronised: one host thread uses it at a time.
+ */
+#ifndef SVGF_H
+#define SVGF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVGF_ABI_VERSION 1u
+
+#if defined(__GNUC__) || defined(__clang__)
+#define SVGF_API __attribute__((visibility("default")))
+#else
+#define SVGF_API
+#endif
+
+typedef enum svgf_status {
+    SVGF_OK = 0,
+    SVGF_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size, pitch, format or parameter */
+    SVGF_ERR_UNSUPPORTED = 2,      /* valid request this build does not implement */
+    SVGF_ERR_OUT_OF_MEMORY = 3,    /* device allocation failed */
+    SVGF_ERR_OVER_BUDGET = 4,      /* stream admission would exceed the HBM budget */
+    SVGF_ERR_NO_STREAM = 5,        /* unknown or closed stream id */
+    SVGF_ERR_CAPACITY = 6,         /* more streams / pixels than the context was created for */
+    SVGF_ERR_HIP = 7,              /* a HIP runtime call failed */
+    SVGF_ERR_NO_DEVICE = 8,        /* no usable gfx950 device */
+    SVGF_ERR_TIMEOUT = 9           /* a bounded wait gave up (batched scheduler) */
+} svgf_status;
+
+/* Algorithm selection.
+ * REFERENCE: the exact semantics of svgf_denoise.slang (same-pixel luminance moments, variance
+ *            from temporal moments only, luminance-only edge stop, sigma passed through the
+ *            alpha channel, output alpha = 1). Planes used: radiance, moments_prev,
+ *            moments_curr, denoised.
+ * CANONICAL: Schied et al. 2017 SVGF (cited at svgf_denoise.slang:18): motion-vector
+ *            reprojection with depth/normal history rejection, colour history, 3x3 spatial
+ *            variance for young history, depth/normal/luminance edge stops, variance
+ *            filtered by squared weights. Uses every plane of svgf_frame. */
+typedef enum svgf_mode { SVGF_MODE_REFERENCE = 0, SVGF_MODE_CANONICAL = 1 } svgf_mode;
+
+/* Storage width of radiance, history colour, moments, wavelet scratch and output.
+ * FP32: every plane float (the Slang CPU-target semantics of the reference).
+ * FP16: those planes half, all arithmetic float (the Vulkan rgba16f / rg16f formats of
+ *       main.cpp:1333-1370). G-buffer and motion stay FP32 in both. */
+typedef enum svgf_storage { SVGF_STORAGE_FP32 = 0, SVGF_STORAGE_FP16 = 1 } svgf_storage;
+
+typedef enum svgf_format {
+    SVGF_FMT_RGBA32F = 1, /* 16 B/px */
+    SVGF_FMT_RGBA16F = 2, /*  8 B/px */
+    SVGF_FMT_RG32F = 3,   /*  8 B/px */
+    SVGF_FMT_RG16F = 4    /*  4 B/px */
+} svgf_format;
+
+/* Flags of svgf_params.flags */
+#define SVGF_FLAG_COMPENSATED_MOMENTS 0x1u /* moments kept as (mean, sqrt(centred 2nd moment)),
+                                              updated by the West/Finch recursion; never forms
+                                              E[x^2]. Default for FP16 storage. */
+
+typedef struct svgf_params {
+    uint32_t iterations;  /* a-trous iterations, strides 1,2,4,...; reference 3 (main.cpp:929) */
+    float phi_luma;       /* luminance edge-stop scale; reference SIGMA_COLOR_BASE = 2
+                             (svgf_denoise.slang:137) */
+    float phi_normal;     /* CANONICAL: exponent of max(0, n.n'); 0 disables */
+    float phi_depth;      /* CANONICAL: depth edge-stop scale; 0 disables */
+    float alpha_color;    /* CANONICAL: colour history blend */
+    float alpha_moments;  /* moments blend; reference ALPHA_MOMENTS = 0.2 (svgf_denoise.slang:144) */
+    float variance_eps;   /* reference VARIANCE_EPSILON = 1e-4 (svgf_denoise.slang:150) */
+    float depth_eps;      /* CANONICAL: additive term of the depth edge-stop denominator */
+    float depth_reject;   /* CANONICAL: history tap rejected when |z'-z| > depth_reject * z */
+    float normal_reject;  /* CANONICAL: history tap rejected when n.n' < normal_reject */
+    uint32_t history_cap; /* CANONICAL: cap of the history length */
+    uint32_t flags;       /* SVGF_FLAG_* */
+} svgf_params;
+
+typedef struct svgf_desc {
+    uint32_t mode;             /* svgf_mode */
+    uint32_t storage;          /* svgf_storage */
+    svgf_params params;
+    uint32_t max_width;        /* largest frame svgf_denoise will see (scratch sizing) */
+    uint32_t max_height;
+    uint32_t max_streams;      /* batched form: resident streams; 0 = single-frame use only */
+    uint32_t max_batch_frames; /* batched form: requests per svgf_denoise_batch call */
+    uint64_t max_batch_pixels; /* batched form: total pixels per svgf_denoise_batch call */
+    uint64_t hbm_budget_bytes; /* batched form: admission budget for stream state; 0 = none */
+    int32_t device;            /* HIP device ordinal */
+    uint32_t batch_flags;      /* SVGF_BATCH_* */
+} svgf_desc;
+
+/* Flags of svgf_desc.batch_flags */
+#define SVGF_BATCH_NO_GRAPH 0x1u /* launch the batched stages directly instead of replaying a
+                                    captured hipGraph */
+#define SVGF_PROFILE_STAGES 0x2u /* record a hipEvent around every stage launch (no graph);
+                                    read back with svgf_stage_times */
+
+typedef struct svgf_plane {
+    void* ptr;            /* device pointer to row 0, pixel 0 */
+    uint64_t pitch_bytes; /* bytes between rows; >= width * pixel size, multiple of pixel size */
+    uint32_t format;      /* svgf_format */
+    uint32_t reserved;
+} svgf_plane;
+
+/* One frame of the single-frame form (caller-owned planes).
+ * Reference binding map (svgf_denoise.slang:61-94, main.cpp:2314-2342):
+ *   radiance = binding 0 lighting_raw, denoised = binding 3, moments_prev = binding 4,
+ *   moments_curr = binding 5. wavelet_a / wavelet_b (bindings 1, 2) are library scratch.
+ * CANONICAL adds gbuffer (normal.xyz, linear depth; RGBA32F), gbuffer_prev, motion (RG32F,
+ *   pixels, previous position = current position + motion) and the colour history
+ *   (rgb, history length in alpha). */
+typedef struct svgf_frame {
+    uint32_t width;
+    uint32_t height;
+    uint32_t history_valid; /* 0 = cold history: *_prev planes are ignored (REFERENCE reads
+                               them as zero, as after main.cpp:1449-1455); 1 = use them */
+    uint32_t reserved;
+    svgf_plane radiance;
+    svgf_plane gbuffer;
+    svgf_plane gbuffer_prev;
+    svgf_plane motion;
+    svgf_plane history_prev;
+    svgf_plane history_curr;
+    svgf_plane moments_prev;
+    svgf_plane moments_curr;
+    svgf_plane denoised;
+} svgf_frame;
+
+/* One request of the batched form: per-frame inputs and output; history is library-resident
+ * per stream (opened with svgf_stream_open). */
+typedef struct svgf_request {
+    uint32_t stream_id;
+    uint32_t reserved;
+    svgf_plane radiance;
+    svgf_plane gbuffer; /* CANONICAL only */
+    svgf_plane motion;  /* CANONICAL only */
+    svgf_plane denoised;
+} svgf_request;
+
+/* Device pointers of a stream's resident state for the frame the NEXT request will read
+ * (history colour, moments, previous G-buffer). For export / inspection. */
+typedef struct svgf_stream_state {
+    uint32_t width;
+    uint32_t height;
+    uint32_t frames;        /* frames denoised since open/reset */
+    uint32_t reserved;
+    svgf_plane history;     /* CANONICAL */
+    svgf_plane moments;
+    svgf_plane gbuffer;     /* CANONICAL */
+} svgf_stream_state;
+
+typedef struct svgf_context svgf_context;
+
+/* --- library ---------------------------------------------------------------------------- */
+SVGF_API uint32_t svgf_abi_version(void);
+SVGF_API const char* svgf_status_str(svgf_status status);
+/* Defaults: REFERENCE = the shader's constants (3 iterations, phi 2, alpha 0.2, eps 1e-4);
+ * CANONICAL = 5 iterations, phi_luma 4, phi_normal 128, phi_depth 1, alpha 0.2/0.2. */
+SVGF_API svgf_status svgf_default_params(uint32_t mode, uint32_t storage, svgf_params* out);
+/* Bytes the library allocates: scratch for a context (at max_width x max_height and the
+ * batch limits) and the resident state of one stream of w x h. */
+SVGF_API svgf_status svgf_query_bytes(const svgf_desc* desc, uint32_t w, uint32_t h,
+                             uint64_t* context_bytes, uint64_t* stream_bytes);
+
+/* --- context (replaces pipeline + descriptor-set creation, main.cpp:2288-2446) ---------- */
+SVGF_API svgf_status svgf_create(const svgf_desc* desc, svgf_context** out);
+SVGF_API svgf_status svgf_destroy(svgf_context* ctx);
+SVGF_API svgf_status svgf_set_params(svgf_context* ctx, const svgf_params* params);
+
+/* --- single frame (replaces the SVGF block of recordFrame, main.cpp:907-947) ------------ */
+SVGF_API svgf_status svgf_denoise(svgf_context* ctx, const svgf_frame* frame, void* hip_stream);
+
+/* --- resident streams (history reset = main.cpp:1570-1603 resize semantics) ------------- */
+SVGF_API svgf_status svgf_stream_open(svgf_context* ctx, uint32_t width, uint32_t height, uint32_t* stream_id);
+SVGF_API svgf_status svgf_stream_close(svgf_context* ctx, uint32_t stream_id);
+SVGF_API svgf_status svgf_stream_reset(svgf_context* ctx, uint32_t stream_id, void* hip_stream);
+SVGF_API svgf_status svgf_stream_state_get(svgf_context* ctx, uint32_t stream_id, svgf_stream_state* out);
+/* Bytes of the HBM budget in use by open streams. */
+SVGF_API svgf_status svgf_budget_used(svgf_context* ctx, uint64_t* bytes);
+
+/* --- batched form: n requests of uneven sizes in one ragged-tile launch sequence -------- */
+SVGF_API svgf_status svgf_denoise_batch(svgf_context* ctx, const svgf_request* requests, uint32_t n,
+                               void* hip_stream);
+
+/* --- per-stage GPU time of the last svgf_denoise / svgf_denoise_batch call --------------
+ * Needs SVGF_PROFILE_STAGES in svgf_desc.batch_flags (hipEvents around each stage launch, the
+ * counterpart of the reference's timestamp pair around the SVGF chain, main.cpp:907, 947).
+ * Waits for that call to finish. Stage order: REFERENCE init, a-trous 0..N-1; CANONICAL
+ * temporal, variance, a-trous 0..N-1. *count receives the number of stages. */
+SVGF_API svgf_status svgf_stage_times(svgf_context* ctx, float* ms, uint32_t capacity, uint32_t* count);
+
+/* --- seeded synthetic G-buffer sequences (host memory; no GPU needed) -------------------- */
+typedef struct svgf_synth_desc {
+    uint32_t width;
+    uint32_t height;
+    uint64_t seed;          /* per-frame noise seed */
+    uint64_t stream;        /* agent-stream key */
+    uint32_t frame;         /* frame index within the sequence (camera position) */
+    uint32_t noise;         /* 0 = exponential (mean 1), 1 = lognormal (sigma 1.5, mean 1) */
+    float pan_px;           /* camera pan per frame in pixels (motion vectors) */
+    float disocclusion;     /* fraction of pixels whose G-buffer is forced inconsistent */
+    uint32_t threads;       /* host threads; 0 = hardware concurrency */
+    uint32_t reserved;
+} svgf_synth_desc;
+/* Fills tightly packed host arrays (any may be NULL): radiance RGBA32F, gbuffer RGBA32F
+ * (normal.xyz, depth), motion RG32F. */
+SVGF_API svgf_status svgf_synth_frame(const svgf_synth_desc* desc, float* radiance, float* gbuffer,
+                             float* motion);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVGF_H */

@@ -1,0 +1,168 @@
+"""ctypes wrapper of the CPU oracle (oracle/svgf_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker. The product path never loads it.
+
+Arrays are numpy, tightly packed, row-major, channels interleaved: colour-like planes are
+(h, w, 4), moments and motion (h, w, 2).
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libsvgf_oracle.so"
+
+FLAG_COMPENSATED = 0x1
+FLAG_AS_REFERENCE = 0x100
+STORE_F32 = 0
+STORE_F16 = 1
+
+
+class OracleParams(ctypes.Structure):
+    """Field order of svgf_params (include/svgf.h) and oracle_params."""
+
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("phi_luma", ctypes.c_float),
+        ("phi_normal", ctypes.c_float),
+        ("phi_depth", ctypes.c_float),
+        ("alpha_color", ctypes.c_float),
+        ("alpha_moments", ctypes.c_float),
+        ("variance_eps", ctypes.c_float),
+        ("depth_eps", ctypes.c_float),
+        ("depth_reject", ctypes.c_float),
+        ("normal_reject", ctypes.c_float),
+        ("history_cap", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+def reference_params(iterations: int = 3) -> OracleParams:
+    """The shader's constants: svgf_denoise.slang:118-150, strides main.cpp:929."""
+    return OracleParams(iterations, 2.0, 0.0, 0.0, 1.0, 0.2, 1e-4, 0.0, 0.0, 0.0, 1, 0)
+
+
+def canonical_params(iterations: int = 5, compensated: bool = False) -> OracleParams:
+    """Canonical SVGF defaults (Schied et al. 2017): phi_l 4, phi_n 128, phi_z 1."""
+    return OracleParams(iterations, 4.0, 128.0, 1.0, 0.2, 0.2, 1e-4, 1e-3, 0.1, 0.9, 32,
+                        FLAG_COMPENSATED if compensated else 0)
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        pp = ctypes.POINTER(OracleParams)
+        L.oracle_r_frame_f32.argtypes = [ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, ctypes.c_int,
+                                         fp, fp, fp, fp, fp, fp, ctypes.c_int]
+        L.oracle_r_frame_f64.argtypes = [ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, dp, dp, dp, dp, dp, dp]
+        L.oracle_c_frame_f64.argtypes = [ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, fp, fp, fp, fp,
+                                         dp, dp, dp, dp, dp, dp, dp]
+        L.oracle_f32_to_f16.argtypes = [ctypes.c_float]
+        L.oracle_f32_to_f16.restype = ctypes.c_uint16
+        L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
+        L.oracle_f16_to_f32.restype = ctypes.c_float
+        L.oracle_round_f16_array.argtypes = [fp, ctypes.c_int64]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def round_f16(a: np.ndarray) -> np.ndarray:
+    """binary16 round-to-nearest-even of a float32 array (the oracle's own conversion)."""
+    out = np.array(a, dtype=np.float32, copy=True, order="C")
+    lib().oracle_round_f16_array(_fp(out), out.size)
+    return out
+
+
+def r_frame_f32(lighting, moments_prev, params=None, store=STORE_F32, history_valid=True, threads=1):
+    """One REFERENCE-mode frame with float arithmetic. Returns (moments_curr, denoised,
+    wavelet_a, wavelet_b)."""
+    p = params or reference_params()
+    lighting = _f32(lighting)
+    h, w = lighting.shape[:2]
+    mp = _f32(moments_prev) if moments_prev is not None else np.zeros((h, w, 2), np.float32)
+    mc = np.zeros((h, w, 2), np.float32)
+    den = np.zeros((h, w, 4), np.float32)
+    wa = np.zeros((h, w, 4), np.float32)
+    wb = np.zeros((h, w, 4), np.float32)
+    rc = lib().oracle_r_frame_f32(w, h, ctypes.byref(p), store, int(bool(history_valid)), _fp(lighting),
+                                  _fp(mp), _fp(mc), _fp(den), _fp(wa), _fp(wb), int(threads))
+    if rc:
+        raise ValueError(f"oracle_r_frame_f32 failed: {rc}")
+    return mc, den, wa, wb
+
+
+def r_frame_f64(lighting, moments_prev, params=None, history_valid=True):
+    """One REFERENCE-mode frame in double. Returns (moments_curr, denoised)."""
+    p = params or reference_params()
+    lighting = _f64(lighting)
+    h, w = lighting.shape[:2]
+    mp = _f64(moments_prev) if moments_prev is not None else np.zeros((h, w, 2))
+    mc = np.zeros((h, w, 2))
+    den = np.zeros((h, w, 4))
+    wa = np.zeros((h, w, 4))
+    wb = np.zeros((h, w, 4))
+    rc = lib().oracle_r_frame_f64(w, h, ctypes.byref(p), int(bool(history_valid)), _dp(lighting), _dp(mp),
+                                  _dp(mc), _dp(den), _dp(wa), _dp(wb))
+    if rc:
+        raise ValueError(f"oracle_r_frame_f64 failed: {rc}")
+    return mc, den
+
+
+def c_frame_f64(radiance, gbuffer, gbuffer_prev, motion, history_prev, moments_prev, params=None,
+                history_valid=True, debug=False):
+    """One CANONICAL frame in double. Returns dict with history, moments, denoised (and
+    temporal / variance stage outputs when debug)."""
+    p = params or canonical_params()
+    radiance = _f32(radiance)
+    h, w = radiance.shape[:2]
+    hist_prev = _f64(history_prev) if history_prev is not None else np.zeros((h, w, 4))
+    mom_prev = _f64(moments_prev) if moments_prev is not None else np.zeros((h, w, 2))
+    out = {
+        "history": np.zeros((h, w, 4)),
+        "moments": np.zeros((h, w, 2)),
+        "denoised": np.zeros((h, w, 4)),
+    }
+    if debug:
+        out["temporal"] = np.zeros((h, w, 4))
+        out["variance"] = np.zeros((h, w, 4))
+    gp = _f32(gbuffer_prev) if gbuffer_prev is not None else np.zeros((h, w, 4), np.float32)
+    rc = lib().oracle_c_frame_f64(
+        w, h, ctypes.byref(p), int(bool(history_valid)), _fp(radiance), _fp(_f32(gbuffer)), _fp(gp),
+        _fp(_f32(motion)), _dp(hist_prev), _dp(mom_prev), _dp(out["history"]), _dp(out["moments"]),
+        _dp(out["denoised"]), _dp(out.get("temporal")), _dp(out.get("variance")))
+    if rc:
+        raise ValueError(f"oracle_c_frame_f64 failed: {rc}")
+    return out

@@ -1,0 +1,82 @@
+/*
+ * svgf_oracle.h — CPU oracle for the SVGF denoiser. TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this
+ * library; it is the checker, never the product path.
+ *
+ * Two restatements live here:
+ *   oracle_r_*  the reference shader src/svgf_denoise.slang:152-275 as written (REFERENCE
+ *               mode), with storage = float (Slang CPU-target semantics), float rounded to
+ *               half at every image store (the Vulkan rgba16f/rg16f images of
+ *               src/main.cpp:1333-1370), or double.
+ *   oracle_c_*  canonical SVGF (Schied et al. 2017, cited at svgf_denoise.slang:18) in
+ *               double precision (CANONICAL mode), with the reference's clamp-to-edge taps
+ *               and tap order, and the history-rejection pattern of
+ *               src/volumetric_filter.slang:171-223.
+ *
+ * Planes are tightly packed row-major arrays with interleaved channels (4 for colour-like
+ * planes, 2 for moments / motion).
+ */
+#ifndef SVGF_ORACLE_H
+#define SVGF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field order and meaning as svgf_params in include/svgf.h. */
+typedef struct oracle_params {
+    uint32_t iterations;
+    float phi_luma;
+    float phi_normal;
+    float phi_depth;
+    float alpha_color;
+    float alpha_moments;
+    float variance_eps;
+    float depth_eps;
+    float depth_reject;
+    float normal_reject;
+    uint32_t history_cap;
+    uint32_t flags;
+} oracle_params;
+
+#define ORACLE_FLAG_COMPENSATED 0x1u /* moments as (mean, sqrt(centred second moment)) */
+#define ORACLE_FLAG_AS_REFERENCE 0x100u /* oracle_c_* with every canonical feature switched
+                                           off: must reproduce oracle_r_f64 */
+
+enum { ORACLE_STORE_F32 = 0, ORACLE_STORE_F16 = 1 };
+
+/* IEEE binary16 round-to-nearest-even (overflow to inf, subnormals kept). */
+uint16_t oracle_f32_to_f16(float f);
+float oracle_f16_to_f32(uint16_t h);
+void oracle_round_f16_array(float* data, int64_t n);
+
+/* One REFERENCE-mode frame. lighting: w*h*4; moments_prev/curr: w*h*2; denoised,
+ * wavelet_a, wavelet_b: w*h*4. history_valid = 0 reads moments_prev as zero.
+ * threads > 1 splits rows statically over pthreads (CPU baseline timing). */
+int oracle_r_frame_f32(int w, int h, const oracle_params* p, int store, int history_valid,
+                       const float* lighting, const float* moments_prev, float* moments_curr,
+                       float* denoised, float* wavelet_a, float* wavelet_b, int threads);
+int oracle_r_frame_f64(int w, int h, const oracle_params* p, int history_valid,
+                       const double* lighting, const double* moments_prev, double* moments_curr,
+                       double* denoised, double* wavelet_a, double* wavelet_b);
+
+/* One CANONICAL frame in double precision.
+ * radiance, gbuffer, gbuffer_prev: w*h*4 float; motion: w*h*2 float (inputs as the GPU sees
+ * them). history_prev/curr: w*h*4 (rgb, history length); moments_prev/curr: w*h*2 (raw
+ * (mu1, mu2), or (mean, sigma) with ORACLE_FLAG_COMPENSATED); denoised: w*h*4.
+ * Optional debug outputs (may be NULL): temporal w*h*4 (colour, history length),
+ * variance w*h*4 (colour, variance) = the input of the first a-trous iteration. */
+int oracle_c_frame_f64(int w, int h, const oracle_params* p, int history_valid,
+                       const float* radiance, const float* gbuffer, const float* gbuffer_prev,
+                       const float* motion, const double* history_prev, const double* moments_prev,
+                       double* history_curr, double* moments_curr, double* denoised,
+                       double* dbg_temporal, double* dbg_variance);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,71 @@
+"""Shared helpers of the test suite: error metrics and sequence drivers (GPU via the C ABI,
+CPU via the oracle)."""
+from __future__ import annotations
+
+import numpy as np
+
+REL_FLOOR = 1e-3  # cd/m^2: relative errors are |a - b| / max(|b|, REL_FLOOR)
+
+
+def rel_err(a, b, floor=REL_FLOOR):
+    """(max, mean) of |a - b| / max(|b|, floor) over every element."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    r = np.abs(a - b) / np.maximum(np.abs(b), floor)
+    return float(r.max()), float(r.mean())
+
+
+def seq_inputs(w, h, frames, *, seed0=0, stream=0, noise=0, pan_px=0.5, disocclusion=0.0, half=False):
+    """Synthetic inputs for `frames` consecutive frames (frame f uses seed seed0 + f)."""
+    import svgf_amd as S
+
+    out = []
+    for f in range(frames):
+        d = S.synth_frame(w, h, seed=seed0 + f, stream=stream, frame=f, noise=noise, pan_px=pan_px,
+                          disocclusion=disocclusion)
+        if half:
+            d["radiance"] = d["radiance"].astype(np.float16).astype(np.float32)
+        out.append(d)
+    return out
+
+
+def oracle_canonical_sequence(inputs, params):
+    """Run the FP64 canonical oracle over a sequence; returns per-frame output dicts."""
+    import oracle as O
+
+    hist = mom = None
+    gprev = None
+    outs = []
+    for f, d in enumerate(inputs):
+        o = O.c_frame_f64(d["radiance"], d["gbuffer"], gprev, d["motion"], hist, mom, params,
+                          history_valid=f > 0, debug=True)
+        hist, mom, gprev = o["history"], o["moments"], d["gbuffer"]
+        outs.append(o)
+    return outs
+
+
+def gpu_canonical_sequence(ctx, inputs, dtype):
+    """Run the GPU canonical form (single-frame C ABI) over a sequence with caller-owned
+    ping-pong planes; returns per-frame numpy dicts (denoised, history, moments)."""
+    import torch
+
+    dev = torch.device("cuda")
+    h, w = inputs[0]["radiance"].shape[:2]
+    hist = [torch.zeros((h, w, 4), dtype=dtype, device=dev) for _ in range(2)]
+    mom = [torch.zeros((h, w, 2), dtype=dtype, device=dev) for _ in range(2)]
+    gb = [None, None]
+    outs = []
+    for f, d in enumerate(inputs):
+        cur, prev = f % 2, (f + 1) % 2
+        rad = torch.from_numpy(d["radiance"]).to(dev).to(dtype)
+        gb[cur] = torch.from_numpy(d["gbuffer"]).to(dev)
+        mv = torch.from_numpy(d["motion"]).to(dev)
+        den = torch.empty((h, w, 4), dtype=dtype, device=dev)
+        ctx.denoise(radiance=rad, gbuffer=gb[cur], gbuffer_prev=gb[prev] if f else None, motion=mv,
+                    history_prev=hist[prev] if f else None, history_curr=hist[cur],
+                    moments_prev=mom[prev] if f else None, moments_curr=mom[cur], denoised=den,
+                    history_valid=f > 0)
+        torch.cuda.synchronize()
+        outs.append({"denoised": den.float().cpu().numpy(), "history": hist[cur].float().cpu().numpy(),
+                     "moments": mom[cur].float().cpu().numpy()})
+    return outs

@@ -1,0 +1,163 @@
+"""CPU-side checks of the C ABI library: it loads, exports every function include/svgf.h
+declares, and its host-only functions (parameters, status strings, sizing, synthetic inputs)
+behave. No kernel is launched here."""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+import svgf_amd as S
+
+
+def test_library_exports_every_header_function():
+    names = S.header_functions()
+    assert len(names) >= 15
+    L = S.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", str(S.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(names) <= exported
+    # nothing but the C ABI is exported
+    assert {e for e in exported if e.startswith("svgf_")} == set(names)
+
+
+def test_abi_version_and_status_strings():
+    L = S.lib()
+    assert L.svgf_abi_version() == 1
+    for code, text in S.STATUS.items():
+        assert L.svgf_status_str(code).decode() == text
+    assert L.svgf_status_str(1234).decode() == "unknown status"
+
+
+def test_default_params_reference_constants():
+    """svgf_denoise.slang:118-150, main.cpp:929."""
+    p = S.default_params(S.MODE_REFERENCE)
+    assert p.iterations == 3
+    assert p.phi_luma == 2.0
+    assert p.alpha_moments == pytest.approx(0.2)
+    assert p.variance_eps == pytest.approx(1e-4)
+    assert p.flags == 0
+    c16 = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
+    assert c16.iterations == 5 and c16.flags & S.FLAG_COMPENSATED
+    assert S.default_params(S.MODE_CANONICAL, S.STORAGE_FP32).flags == 0
+
+
+def test_default_params_rejects_bad_input():
+    p = S.Params()
+    assert S.lib().svgf_default_params(7, 0, ctypes.byref(p)) == 1
+    assert S.lib().svgf_default_params(0, 0, None) == 1
+
+
+def test_query_bytes():
+    d = S.Desc(S.MODE_CANONICAL, S.STORAGE_FP16, S.default_params(1, 1), 1280, 720, 4, 8, 4_000_000, 0, 0, 0)
+    ctx_b, st_b = ctypes.c_uint64(), ctypes.c_uint64()
+    assert S.lib().svgf_query_bytes(ctypes.byref(d), 1280, 720, ctypes.byref(ctx_b), ctypes.byref(st_b)) == 0
+    px = 1280 * 720
+    # history 2 x rgba16f + moments 2 x rg16f + G copy 2 x rgba32f
+    assert st_b.value >= px * (2 * 8 + 2 * 4 + 2 * 16)
+    assert st_b.value < px * (2 * 8 + 2 * 4 + 2 * 16) + 6 * 256
+    d.mode = S.MODE_REFERENCE
+    d.storage = S.STORAGE_FP32
+    assert S.lib().svgf_query_bytes(ctypes.byref(d), 640, 360, None, ctypes.byref(st_b)) == 0
+    assert st_b.value >= 640 * 360 * 2 * 8
+
+
+def test_create_without_gpu_fails_cleanly():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    d = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, S.default_params(0), 64, 64, 0, 0, 0, 0, 0, 0)
+    h = ctypes.c_void_p()
+    rc = S.lib().svgf_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == 8 and not h.value
+
+
+def test_create_rejects_bad_descriptors():
+    h = ctypes.c_void_p()
+    L = S.lib()
+    bad = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, S.default_params(0), 0, 64, 0, 0, 0, 0, 0, 0)
+    assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == 1
+    bad = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, S.default_params(0), 64, 64, 0, 0, 0, 0, 0, 0x80)
+    assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == 1
+    p = S.default_params(0)
+    p.iterations = 0
+    bad = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, p, 64, 64, 0, 0, 0, 0, 0, 0)
+    assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == 1
+    p = S.default_params(0)
+    p.flags = S.FLAG_COMPENSATED  # compensated moments are not the reference's semantics
+    bad = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, p, 64, 64, 0, 0, 0, 0, 0, 0)
+    assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == 2
+    assert L.svgf_create(None, ctypes.byref(h)) == 1
+    assert L.svgf_destroy(None) == 1
+
+
+def test_null_context_calls_fail_cleanly():
+    L = S.lib()
+    sid = ctypes.c_uint32()
+    assert L.svgf_denoise(None, None, None) == 1
+    assert L.svgf_stream_open(None, 8, 8, ctypes.byref(sid)) == 1
+    assert L.svgf_stream_close(None, 0) == 1
+    assert L.svgf_denoise_batch(None, None, 0, None) == 1
+    n = ctypes.c_uint32()
+    assert L.svgf_stage_times(None, None, 0, ctypes.byref(n)) == 1
+
+
+# ------------------------------------------------------------------------------------------
+# synthetic inputs
+# ------------------------------------------------------------------------------------------
+def test_synth_is_deterministic_and_thread_independent():
+    a = S.synth_frame(97, 41, seed=3, stream=2, frame=5, noise=1, pan_px=1.3, disocclusion=0.1, threads=1)
+    b = S.synth_frame(97, 41, seed=3, stream=2, frame=5, noise=1, pan_px=1.3, disocclusion=0.1, threads=7)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+    c = S.synth_frame(97, 41, seed=4, stream=2, frame=5, noise=1, pan_px=1.3, disocclusion=0.1)
+    assert not np.array_equal(a["radiance"], c["radiance"])
+    # the scene (normals, depth) does not depend on the noise seed outside forced disocclusions
+    e = S.synth_frame(97, 41, seed=4, stream=2, frame=5, noise=1, pan_px=1.3)
+    f = S.synth_frame(97, 41, seed=3, stream=2, frame=5, noise=1, pan_px=1.3)
+    assert np.array_equal(e["gbuffer"], f["gbuffer"])
+
+
+def test_synth_ranges():
+    d = S.synth_frame(640, 360, seed=0, frame=0, noise=0, pan_px=0.5)
+    g = d["gbuffer"]
+    assert g[..., 3].min() >= 0.1 - 1e-6 and g[..., 3].max() <= 100.0 + 1e-4
+    np.testing.assert_allclose(np.linalg.norm(g[..., :3], axis=-1), 1.0, rtol=1e-6)
+    assert np.all(d["motion"][..., 0] == 0.5) and np.all(d["motion"][..., 1] == 0.0)
+    rad = d["radiance"]
+    assert rad[..., :3].min() >= 0.0 and np.all(rad[..., 3] == 1.0)
+    # exponential noise with mean 1 over a 0-10 gradient: mean radiance ~ mean gradient
+    base = S.synth_frame(640, 360, seed=0, frame=0, noise=0, pan_px=0.5)["radiance"]
+    assert 2.0 < base[..., 0].mean() < 8.0
+
+
+def test_synth_lognormal_mean_one_and_fp16_range():
+    d = S.synth_frame(512, 512, seed=1, frame=0, noise=1)
+    r = d["radiance"][..., 0].astype(np.float64)
+    d0 = S.synth_frame(512, 512, seed=1, frame=0, noise=0)
+    ratio = r.sum() / d0["radiance"][..., 0].astype(np.float64).sum()
+    assert 0.85 < ratio < 1.15  # both noises have mean 1
+    assert r.max() < 65504.0
+
+
+def test_synth_pan_motion_is_consistent():
+    """Frame f at x shows what frame f-1 shows at x + pan (integer pan: exact)."""
+    a = S.synth_frame(64, 16, seed=0, frame=3, pan_px=2.0)
+    b = S.synth_frame(64, 16, seed=0, frame=4, pan_px=2.0)
+    np.testing.assert_array_equal(b["gbuffer"][:, :-2], a["gbuffer"][:, 2:])
+
+
+def test_synth_disocclusion_fraction():
+    d = S.synth_frame(400, 300, seed=0, frame=2, pan_px=0.0, disocclusion=0.1)
+    base = S.synth_frame(400, 300, seed=0, frame=2, pan_px=0.0, disocclusion=0.0)
+    frac = np.mean(d["gbuffer"][..., 3] != base["gbuffer"][..., 3])
+    assert 0.09 < frac < 0.11

@@ -1,0 +1,241 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the same seeded inputs.
+
+REFERENCE mode vs the float restatement of src/svgf_denoise.slang: <= 1e-5 relative error
+(FP32 storage). FP16 storage vs the half-store restatement: within one half ulp. CANONICAL mode
+vs the FP64 oracle: tolerances stated per test (FP32 arithmetic against binary64).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from svgf_testutil import gpu_canonical_sequence, oracle_canonical_sequence, rel_err, seq_inputs
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def S():
+    import svgf_amd
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    svgf_amd.lib()
+    return svgf_amd
+
+
+def _ref_gpu(S, ctx, lighting, moments_prev, dtype, history_valid):
+    dev = torch.device("cuda")
+    h, w = lighting.shape[:2]
+    rad = torch.from_numpy(lighting).to(dev).to(dtype)
+    den = torch.empty((h, w, 4), dtype=dtype, device=dev)
+    mom = torch.empty((h, w, 2), dtype=dtype, device=dev)
+    mp = torch.from_numpy(moments_prev).to(dev).to(dtype) if moments_prev is not None else None
+    ctx.denoise(radiance=rad, denoised=den, moments_curr=mom, moments_prev=mp, history_valid=history_valid)
+    torch.cuda.synchronize()
+    return den.float().cpu().numpy(), mom.float().cpu().numpy()
+
+
+@pytest.mark.parametrize("iterations", [3, 5])
+def test_reference_fp32_config0(S, oracle_lib, iterations):
+    """640x360: history = a frame from seed 1, current frame from seed 0 (alpha 0.2)."""
+    O = oracle_lib
+    w, h = 640, 360
+    prev = S.synth_frame(w, h, seed=1, frame=0, pan_px=0.5)["radiance"]
+    cur = S.synth_frame(w, h, seed=0, frame=1, pan_px=0.5)["radiance"]
+    p = S.default_params(S.MODE_REFERENCE)
+    p.iterations = iterations
+    op = O.reference_params(iterations)
+    with S.Context(S.MODE_REFERENCE, S.STORAGE_FP32, params=p, max_width=w, max_height=h) as ctx:
+        d0, m0 = _ref_gpu(S, ctx, prev, None, torch.float32, False)
+        d1, m1 = _ref_gpu(S, ctx, cur, m0, torch.float32, True)
+    om0, od0, _, _ = O.r_frame_f32(prev, None, op, history_valid=False)
+    om1, od1, _, _ = O.r_frame_f32(cur, om0, op, history_valid=True)
+    for a, b, name in ((d0, od0, "denoised[0]"), (m0, om0, "moments[0]"), (d1, od1, "denoised[1]"),
+                       (m1, om1, "moments[1]")):
+        mx, mean = rel_err(a, b)
+        print(f"{name}: max rel {mx:.3e} mean {mean:.3e}")
+        assert mx <= 1e-5, name
+    assert np.all(d1[..., 3] == 1.0)
+
+
+def test_reference_fp16_storage(S, oracle_lib):
+    """The Vulkan rgba16f/rg16f form vs the restatement that rounds every store to half."""
+    O = oracle_lib
+    w, h = 320, 180
+    prev = O.round_f16(S.synth_frame(w, h, seed=1, frame=0)["radiance"])
+    cur = O.round_f16(S.synth_frame(w, h, seed=0, frame=1)["radiance"])
+    op = O.reference_params(3)
+    with S.Context(S.MODE_REFERENCE, S.STORAGE_FP16, max_width=w, max_height=h) as ctx:
+        d0, m0 = _ref_gpu(S, ctx, prev, None, torch.float16, False)
+        d1, m1 = _ref_gpu(S, ctx, cur, m0, torch.float16, True)
+    om0, od0, _, _ = O.r_frame_f32(prev, None, op, store=O.STORE_F16, history_valid=False)
+    om1, od1, _, _ = O.r_frame_f32(cur, om0, op, store=O.STORE_F16, history_valid=True)
+    for a, b, name in ((d0, od0, "denoised[0]"), (m0, om0, "moments[0]"), (d1, od1, "denoised[1]"),
+                       (m1, om1, "moments[1]")):
+        mx, mean = rel_err(a, b)
+        print(f"{name}: max rel {mx:.3e} mean {mean:.3e}")
+        assert mx <= 2.0 ** -10 + 1e-6, name  # one half ulp of disagreement on a rounding tie
+        assert mean <= 1e-5, name
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 9), (9, 2), (65, 5), (67, 131)])
+def test_reference_edge_sizes(S, oracle_lib, shape):
+    O = oracle_lib
+    w, h = shape
+    cur = S.synth_frame(w, h, seed=3, frame=0)["radiance"]
+    with S.Context(S.MODE_REFERENCE, S.STORAGE_FP32, max_width=w, max_height=h) as ctx:
+        d, m = _ref_gpu(S, ctx, cur, None, torch.float32, False)
+    om, od, _, _ = O.r_frame_f32(cur, None, O.reference_params(3), history_valid=False)
+    assert rel_err(d, od)[0] <= 1e-5
+    assert rel_err(m, om)[0] <= 1e-5
+
+
+def test_canonical_fp32_sequence(S, oracle_lib):
+    """6 frames, fractional pan, 10% forced disocclusion: FP32 GPU vs FP64 oracle."""
+    O = oracle_lib
+    w, h = 200, 120
+    inputs = seq_inputs(w, h, 6, pan_px=1.37, disocclusion=0.1)
+    op = O.canonical_params(5)
+    ref = oracle_canonical_sequence(inputs, op)
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP32, max_width=w, max_height=h) as ctx:
+        got = gpu_canonical_sequence(ctx, inputs, torch.float32)
+    for f, (g, r) in enumerate(zip(got, ref)):
+        dm, dmean = rel_err(g["denoised"][..., :3], r["denoised"][..., :3])
+        hm, _ = rel_err(g["history"], r["history"])
+        print(f"frame {f}: denoised max {dm:.3e} mean {dmean:.3e}; history max {hm:.3e}")
+        assert dm <= 2e-4 and dmean <= 1e-6, f
+        assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), "history length differs"
+
+
+def test_canonical_fp16_compensated_sequence(S, oracle_lib):
+    """Precision form: FP16 storage, compensated moments, lognormal noise, vs FP64 oracle."""
+    O = oracle_lib
+    w, h = 200, 120
+    inputs = seq_inputs(w, h, 6, noise=1, pan_px=1.37, disocclusion=0.1, half=True)
+    op = O.canonical_params(5, compensated=True)
+    ref = oracle_canonical_sequence(inputs, op)
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=w, max_height=h) as ctx:
+        got = gpu_canonical_sequence(ctx, inputs, torch.float16)
+    for f, (g, r) in enumerate(zip(got, ref)):
+        dm, dmean = rel_err(g["denoised"][..., :3], r["denoised"][..., :3])
+        print(f"frame {f}: denoised max {dm:.3e} mean {dmean:.3e}")
+        assert np.isfinite(g["denoised"]).all()
+        assert dmean <= 1e-3, f
+
+
+def _ragged_shapes():
+    return [(1, 1), (2, 9), (65, 5), (96, 96), (130, 70), (64, 4)]
+
+
+@pytest.mark.parametrize("mode", ["reference", "canonical"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_matches_single(S, mode, graph):
+    """Each stream's result is bit-identical alone (single-frame form) and in a batch."""
+    m = S.MODE_REFERENCE if mode == "reference" else S.MODE_CANONICAL
+    st = S.STORAGE_FP16
+    dt = torch.float16
+    dev = torch.device("cuda")
+    shapes = _ragged_shapes()
+    frames = 3
+    inputs = [seq_inputs(w, h, frames, seed0=10 * i, stream=i, pan_px=0.7, disocclusion=0.1) for i, (w, h) in
+              enumerate(shapes)]
+    maxw = max(w for w, _ in shapes)
+    maxh = max(h for _, h in shapes)
+    npx = sum(w * h for w, h in shapes)
+    flags = 0 if graph else S.BATCH_NO_GRAPH
+    with S.Context(m, st, max_width=maxw, max_height=maxh, max_streams=8, max_batch_frames=8,
+                   max_batch_pixels=npx, batch_flags=flags) as bctx:
+        sids = [bctx.stream_open(w, h) for w, h in shapes]
+        batched = [[None] * frames for _ in shapes]
+        keep = []
+        for f in range(frames):
+            items = []
+            for i, (w, h) in enumerate(shapes):
+                d = inputs[i][f]
+                it = {"stream_id": sids[i], "radiance": torch.from_numpy(d["radiance"]).to(dev).to(dt),
+                      "denoised": torch.empty((h, w, 4), dtype=dt, device=dev)}
+                if m == S.MODE_CANONICAL:
+                    it["gbuffer"] = torch.from_numpy(d["gbuffer"]).to(dev)
+                    it["motion"] = torch.from_numpy(d["motion"]).to(dev)
+                items.append(it)
+            bctx.denoise_batch(items)
+            keep.append(items)
+            for i, it in enumerate(items):
+                batched[i][f] = it["denoised"]
+        torch.cuda.synchronize()
+    with S.Context(m, st, max_width=maxw, max_height=maxh) as sctx:
+        for i, (w, h) in enumerate(shapes):
+            if m == S.MODE_CANONICAL:
+                alone = gpu_canonical_sequence(sctx, inputs[i], dt)
+                for f in range(frames):
+                    assert np.array_equal(alone[f]["denoised"], batched[i][f].float().cpu().numpy()), (i, f)
+            else:
+                mom = [torch.zeros((h, w, 2), dtype=dt, device=dev) for _ in range(2)]
+                for f in range(frames):
+                    d = inputs[i][f]
+                    rad = torch.from_numpy(d["radiance"]).to(dev).to(dt)
+                    den = torch.empty((h, w, 4), dtype=dt, device=dev)
+                    sctx.denoise(radiance=rad, denoised=den, moments_curr=mom[f % 2],
+                                 moments_prev=mom[(f + 1) % 2] if f else None, history_valid=f > 0)
+                    torch.cuda.synchronize()
+                    assert torch.equal(den, batched[i][f]), (i, f)
+
+
+def test_stream_reset_is_cold(S):
+    """svgf_stream_reset gives the same result as a freshly opened stream."""
+    dev = torch.device("cuda")
+    w, h = 70, 33
+    inputs = seq_inputs(w, h, 3, pan_px=0.5)
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP32, max_width=w, max_height=h, max_streams=2,
+                   max_batch_frames=2, max_batch_pixels=2 * w * h) as ctx:
+        a = ctx.stream_open(w, h)
+        outs = []
+        for f in (0, 1, 2):
+            if f == 2:
+                ctx.stream_reset(a)
+            d = inputs[f]
+            den = torch.empty((h, w, 4), device=dev)
+            ctx.denoise_batch([{"stream_id": a, "radiance": torch.from_numpy(d["radiance"]).to(dev),
+                                "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                                "motion": torch.from_numpy(d["motion"]).to(dev), "denoised": den}])
+            outs.append(den)
+        b = ctx.stream_open(w, h)
+        d = inputs[2]
+        den_b = torch.empty((h, w, 4), device=dev)
+        ctx.denoise_batch([{"stream_id": b, "radiance": torch.from_numpy(d["radiance"]).to(dev),
+                            "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                            "motion": torch.from_numpy(d["motion"]).to(dev), "denoised": den_b}])
+        torch.cuda.synchronize()
+        assert torch.equal(outs[2], den_b)
+        assert not torch.equal(outs[1], den_b)
+
+
+def test_invalid_arguments(S):
+    dev = torch.device("cuda")
+    w, h = 32, 16
+    with S.Context(S.MODE_REFERENCE, S.STORAGE_FP32, max_width=w, max_height=h) as ctx:
+        rad = torch.zeros((h, w, 4), device=dev)
+        den = torch.zeros((h, w, 4), device=dev)
+        mom = torch.zeros((h, w, 2), device=dev)
+        with pytest.raises(S.SvgfError) as e:  # wrong storage format
+            ctx.denoise(radiance=rad.half(), denoised=den, moments_curr=mom, history_valid=False)
+        assert e.value.status == 1
+        with pytest.raises(S.SvgfError) as e:  # missing moments_prev with history
+            ctx.denoise(radiance=rad, denoised=den, moments_curr=mom, history_valid=True)
+        assert e.value.status == 1
+        big = torch.zeros((h, 2 * w, 4), device=dev)
+        with pytest.raises(S.SvgfError) as e:  # larger than the context
+            ctx.denoise(radiance=big, denoised=big, moments_curr=torch.zeros((h, 2 * w, 2), device=dev),
+                        history_valid=False)
+        assert e.value.status == 6
+        with pytest.raises(S.SvgfError) as e:  # batched form not configured
+            ctx.denoise_batch([{"stream_id": 0, "radiance": rad, "denoised": den}])
+        assert e.value.status in (2, 5)
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=w, max_height=h, max_streams=2,
+                   max_batch_frames=2, max_batch_pixels=w * h, hbm_budget_bytes=1) as ctx:
+        with pytest.raises(S.SvgfError) as e:
+            ctx.stream_open(w, h)
+        assert e.value.status == 4

@@ -1,0 +1,662 @@
+// svgf_api.hip — the C ABI of include/svgf.h over HIP.
+//
+// Replaces TronGrid's SVGF host glue: pipeline / descriptor creation (src/main.cpp:2288-2446)
+// becomes svgf_create; the per-frame dispatch block of recordFrame (src/main.cpp:907-947)
+// becomes svgf_denoise; image allocation + zero-init + resize reset (src/main.cpp:1326-1496,
+// 1570-1603) become caller-owned planes (single frame) or library-resident streams (batched).
+#include "../../include/svgf.h"
+#include "svgf_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace svgf;
+
+namespace {
+
+constexpr int BATCH_SLOTS = 3;
+constexpr size_t ALIGN = 256;
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+uint32_t bytes_of(uint32_t fmt)
+{
+    switch (fmt) {
+    case SVGF_FMT_RGBA32F: return 16;
+    case SVGF_FMT_RGBA16F: return 8;
+    case SVGF_FMT_RG32F: return 8;
+    case SVGF_FMT_RG16F: return 4;
+    default: return 0;
+    }
+}
+
+struct StreamSlot {
+    bool open = false;
+    uint32_t w = 0, h = 0;
+    uint32_t frames = 0;
+    int parity = 0;  // state[parity] holds what the next frame reads
+    void* block = nullptr;
+    uint64_t bytes = 0;
+    PlaneDev hist[2]{}, mom[2]{}, gbuf[2]{};
+};
+
+struct BatchSlot {
+    void* staging = nullptr;  // pinned host copy of the table
+    void* table = nullptr;    // device table
+    void* arena = nullptr;    // per-frame scratch planes
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+struct svgf_context {
+    svgf_desc desc{};
+    KParams kp{};
+    Storage st = Storage::F32;
+    int mode = SVGF_MODE_REFERENCE;
+    uint32_t s4 = 16, s2 = 8;  // storage bytes of colour-like and moment planes
+
+    void* scratch = nullptr;
+    PlaneDev wa{}, wb{}, ctmp{}, nlen{};
+
+    std::vector<StreamSlot> streams;
+    uint64_t budget_used = 0;
+
+    BatchSlot slots[BATCH_SLOTS];
+    int next_slot = 0;
+    size_t table_bytes = 0, starts_off = 0, frames_off = 0, arena_bytes = 0;
+    int grid_blocks = 0;
+    hipStream_t capture_stream = nullptr;
+
+    // SVGF_PROFILE_STAGES
+    hipEvent_t ev[MAX_STAGES + 1]{};
+    int ev_stages = 0;
+};
+
+namespace {
+
+svgf_status validate_params(const svgf_params& p, uint32_t mode)
+{
+    if (p.iterations < 1 || p.iterations > 10) return SVGF_ERR_INVALID_ARGUMENT;
+    auto finite_nonneg = [](float v) { return v >= 0.0f && v < 3.0e38f; };
+    if (!finite_nonneg(p.phi_luma) || !finite_nonneg(p.alpha_moments) || p.alpha_moments > 1.0f) return SVGF_ERR_INVALID_ARGUMENT;
+    if (!(p.variance_eps > 0.0f) || !finite_nonneg(p.variance_eps)) return SVGF_ERR_INVALID_ARGUMENT;
+    if (mode == SVGF_MODE_CANONICAL) {
+        if (!finite_nonneg(p.phi_normal) || !finite_nonneg(p.phi_depth) || !finite_nonneg(p.alpha_color) ||
+            p.alpha_color > 1.0f || !(p.depth_eps > 0.0f) || !finite_nonneg(p.depth_reject) ||
+            !(p.normal_reject >= -1.0f && p.normal_reject <= 1.0f) || p.history_cap < 1 || p.history_cap > 255)
+            return SVGF_ERR_INVALID_ARGUMENT;
+    }
+    if (p.flags & ~SVGF_FLAG_COMPENSATED_MOMENTS) return SVGF_ERR_INVALID_ARGUMENT;
+    if (mode == SVGF_MODE_REFERENCE && (p.flags & SVGF_FLAG_COMPENSATED_MOMENTS)) return SVGF_ERR_UNSUPPORTED;
+    return SVGF_OK;
+}
+
+KParams to_kparams(const svgf_params& p)
+{
+    KParams k{};
+    k.phi_luma = p.phi_luma;
+    k.phi_normal = p.phi_normal;
+    k.phi_depth = p.phi_depth;
+    k.alpha_color = p.alpha_color;
+    k.alpha_moments = p.alpha_moments;
+    k.variance_eps = p.variance_eps;
+    k.depth_eps = p.depth_eps;
+    k.depth_reject = p.depth_reject;
+    k.normal_reject = p.normal_reject;
+    k.history_cap = (int32_t)p.history_cap;
+    k.iterations = (int32_t)p.iterations;
+    k.compensated = (p.flags & SVGF_FLAG_COMPENSATED_MOMENTS) ? 1 : 0;
+    return k;
+}
+
+svgf_status check_plane(const svgf_plane& pl, uint32_t want_fmt, uint32_t w)
+{
+    if (!pl.ptr) return SVGF_ERR_INVALID_ARGUMENT;
+    if (pl.format != want_fmt) return SVGF_ERR_INVALID_ARGUMENT;
+    uint32_t b = bytes_of(want_fmt);
+    if ((reinterpret_cast<uintptr_t>(pl.ptr) % b) != 0) return SVGF_ERR_INVALID_ARGUMENT;
+    if (pl.pitch_bytes < (uint64_t)w * b || (pl.pitch_bytes % b) != 0) return SVGF_ERR_INVALID_ARGUMENT;
+    return SVGF_OK;
+}
+
+PlaneDev dev_plane(const svgf_plane& p) { return PlaneDev{static_cast<char*>(p.ptr), p.pitch_bytes}; }
+
+uint32_t fmt4(const svgf_context* c) { return c->st == Storage::F32 ? SVGF_FMT_RGBA32F : SVGF_FMT_RGBA16F; }
+uint32_t fmt2(const svgf_context* c) { return c->st == Storage::F32 ? SVGF_FMT_RG32F : SVGF_FMT_RG16F; }
+
+uint64_t stream_state_bytes(uint32_t mode, uint32_t s4, uint32_t s2, uint32_t w, uint32_t h)
+{
+    uint64_t px = (uint64_t)w * h;
+    uint64_t b = 2 * align_up(px * s2, ALIGN);
+    if (mode == SVGF_MODE_CANONICAL) b += 2 * align_up(px * s4, ALIGN) + 2 * align_up(px * 16, ALIGN);
+    return b;
+}
+
+// Scratch of one frame in a batch or of the single-frame form: wa, wb, ctmp (storage x4) and
+// the uint8 history length.
+uint64_t frame_scratch_bytes(uint32_t s4, uint64_t px)
+{
+    return 3 * align_up(px * s4, ALIGN) + align_up(px, ALIGN);
+}
+
+void carve_scratch(char* base, uint32_t s4, uint32_t w, uint32_t h, PlaneDev& wa, PlaneDev& wb, PlaneDev& ctmp,
+                   PlaneDev& nlen)
+{
+    uint64_t px = (uint64_t)w * h;
+    size_t p4 = align_up(px * s4, ALIGN);
+    wa = PlaneDev{base, (uint64_t)w * s4};
+    wb = PlaneDev{base + p4, (uint64_t)w * s4};
+    ctmp = PlaneDev{base + 2 * p4, (uint64_t)w * s4};
+    nlen = PlaneDev{base + 3 * p4, (uint64_t)w};
+}
+
+uint32_t tiles_of(uint32_t w, uint32_t h)
+{
+    return ((w + TILE_W - 1) / TILE_W) * ((h + TILE_H - 1) / TILE_H);
+}
+
+void free_slot(BatchSlot& s)
+{
+    if (s.exec) (void)hipGraphExecDestroy(s.exec);
+    if (s.graph) (void)hipGraphDestroy(s.graph);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.staging) (void)hipHostFree(s.staging);
+    if (s.table) (void)hipFree(s.table);
+    if (s.arena) (void)hipFree(s.arena);
+    s = BatchSlot{};
+}
+
+void drop_graphs(svgf_context* c)
+{
+    for (auto& s : c->slots) {
+        if (s.pending) {
+            (void)hipEventSynchronize(s.done);
+            s.pending = false;
+        }
+        if (s.exec) (void)hipGraphExecDestroy(s.exec);
+        if (s.graph) (void)hipGraphDestroy(s.graph);
+        s.exec = nullptr;
+        s.graph = nullptr;
+    }
+}
+
+void destroy_context(svgf_context* c)
+{
+    (void)hipDeviceSynchronize();
+    for (auto& s : c->slots) free_slot(s);
+    for (auto& s : c->streams)
+        if (s.block) (void)hipFree(s.block);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete c;
+}
+
+int stage_count(const svgf_context* c) { return c->kp.iterations + (c->mode == SVGF_MODE_REFERENCE ? 1 : 2); }
+
+hipEvent_t* profile_events(svgf_context* c)
+{
+    if (!(c->desc.batch_flags & SVGF_PROFILE_STAGES)) return nullptr;
+    c->ev_stages = stage_count(c);
+    return c->ev;
+}
+
+hipError_t enqueue_batch(svgf_context* c, BatchSlot& s, hipStream_t stream, hipEvent_t* ev = nullptr)
+{
+    hipError_t e = hipMemcpyAsync(s.table, s.staging, c->table_bytes, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
+    char* t = static_cast<char*>(s.table);
+    BatchTable bt{reinterpret_cast<const uint32_t*>(t), reinterpret_cast<const uint32_t*>(t + c->starts_off),
+                  reinterpret_cast<const FrameDesc*>(t + c->frames_off)};
+    if (c->mode == SVGF_MODE_REFERENCE) return launch_reference_batch(bt, c->kp, c->st, c->grid_blocks, stream, ev);
+    return launch_canonical_batch(bt, c->kp, c->st, c->grid_blocks, stream, ev);
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t svgf_abi_version(void) { return SVGF_ABI_VERSION; }
+
+const char* svgf_status_str(svgf_status s)
+{
+    switch (s) {
+    case SVGF_OK: return "ok";
+    case SVGF_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case SVGF_ERR_UNSUPPORTED: return "unsupported";
+    case SVGF_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case SVGF_ERR_OVER_BUDGET: return "over HBM budget";
+    case SVGF_ERR_NO_STREAM: return "no such stream";
+    case SVGF_ERR_CAPACITY: return "over context capacity";
+    case SVGF_ERR_HIP: return "HIP runtime error";
+    case SVGF_ERR_NO_DEVICE: return "no gfx950 device";
+    case SVGF_ERR_TIMEOUT: return "timeout";
+    }
+    return "unknown status";
+}
+
+svgf_status svgf_default_params(uint32_t mode, uint32_t storage, svgf_params* out)
+{
+    if (!out || mode > SVGF_MODE_CANONICAL || storage > SVGF_STORAGE_FP16) return SVGF_ERR_INVALID_ARGUMENT;
+    svgf_params p{};
+    if (mode == SVGF_MODE_REFERENCE) {
+        // svgf_denoise.slang:118-150 and main.cpp:929
+        p.iterations = 3;
+        p.phi_luma = 2.0f;
+        p.alpha_moments = 0.2f;
+        p.variance_eps = 1e-4f;
+        p.alpha_color = 1.0f;
+        p.history_cap = 1;
+        p.depth_eps = 1e-3f;
+    } else {
+        p.iterations = 5;
+        p.phi_luma = 4.0f;
+        p.phi_normal = 128.0f;
+        p.phi_depth = 1.0f;
+        p.alpha_color = 0.2f;
+        p.alpha_moments = 0.2f;
+        p.variance_eps = 1e-4f;
+        p.depth_eps = 1e-3f;
+        p.depth_reject = 0.1f;
+        p.normal_reject = 0.9f;
+        p.history_cap = 32;
+        p.flags = storage == SVGF_STORAGE_FP16 ? SVGF_FLAG_COMPENSATED_MOMENTS : 0u;
+    }
+    *out = p;
+    return SVGF_OK;
+}
+
+svgf_status svgf_query_bytes(const svgf_desc* d, uint32_t w, uint32_t h, uint64_t* context_bytes, uint64_t* stream_bytes)
+{
+    if (!d || d->mode > SVGF_MODE_CANONICAL || d->storage > SVGF_STORAGE_FP16) return SVGF_ERR_INVALID_ARGUMENT;
+    uint32_t s4 = d->storage == SVGF_STORAGE_FP32 ? 16 : 8, s2 = d->storage == SVGF_STORAGE_FP32 ? 8 : 4;
+    if (context_bytes) {
+        uint64_t b = frame_scratch_bytes(s4, (uint64_t)d->max_width * d->max_height);
+        if (d->max_batch_frames) {
+            uint64_t max_tiles = d->max_batch_pixels / (TILE_W * TILE_H) +
+                                 (uint64_t)d->max_batch_frames * (d->max_width / TILE_W + d->max_height / TILE_H + 2);
+            (void)max_tiles;
+            uint64_t table = align_up(64 + 4ull * (d->max_batch_frames + 1), ALIGN) +
+                             sizeof(FrameDesc) * (uint64_t)d->max_batch_frames;
+            uint64_t arena = d->max_batch_pixels * (3ull * s4 + 1) + (uint64_t)d->max_batch_frames * 4 * ALIGN;
+            b += BATCH_SLOTS * (table + arena);
+        }
+        *context_bytes = b;
+    }
+    if (stream_bytes) *stream_bytes = stream_state_bytes(d->mode, s4, s2, w, h);
+    return SVGF_OK;
+}
+
+svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
+{
+    if (!d || !out) return SVGF_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (d->mode > SVGF_MODE_CANONICAL || d->storage > SVGF_STORAGE_FP16) return SVGF_ERR_INVALID_ARGUMENT;
+    if (d->max_width < 1 || d->max_height < 1 || d->max_width > 16384 || d->max_height > 16384)
+        return SVGF_ERR_INVALID_ARGUMENT;
+    if (d->max_batch_frames > MAX_BATCH_FRAMES) return SVGF_ERR_INVALID_ARGUMENT;
+    if (d->max_batch_frames && d->max_batch_pixels < 1) return SVGF_ERR_INVALID_ARGUMENT;
+    if (d->batch_flags & ~(SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES)) return SVGF_ERR_INVALID_ARGUMENT;
+    svgf_status ps = validate_params(d->params, d->mode);
+    if (ps != SVGF_OK) return ps;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || d->device < 0 || d->device >= ndev) return SVGF_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d->device) != hipSuccess) return SVGF_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SVGF_ERR_NO_DEVICE;
+    DeviceGuard guard(d->device);
+
+    svgf_context* c = new (std::nothrow) svgf_context();
+    if (!c) return SVGF_ERR_OUT_OF_MEMORY;
+    c->desc = *d;
+    c->mode = (int)d->mode;
+    c->st = d->storage == SVGF_STORAGE_FP32 ? Storage::F32 : Storage::F16;
+    c->s4 = c->st == Storage::F32 ? 16 : 8;
+    c->s2 = c->st == Storage::F32 ? 8 : 4;
+    c->kp = to_kparams(d->params);
+    c->grid_blocks = std::max(1, prop.multiProcessorCount) * 8;
+
+    uint64_t sb = frame_scratch_bytes(c->s4, (uint64_t)d->max_width * d->max_height);
+    if (hipMalloc(&c->scratch, sb) != hipSuccess) {
+        c->scratch = nullptr;
+        destroy_context(c);
+        return SVGF_ERR_OUT_OF_MEMORY;
+    }
+    carve_scratch(static_cast<char*>(c->scratch), c->s4, d->max_width, d->max_height, c->wa, c->wb, c->ctmp, c->nlen);
+    if (d->batch_flags & SVGF_PROFILE_STAGES) {
+        for (auto& e : c->ev)
+            if (hipEventCreate(&e) != hipSuccess) {
+                e = nullptr;
+                destroy_context(c);
+                return SVGF_ERR_HIP;
+            }
+    }
+
+    if (d->max_streams) c->streams.resize(d->max_streams);
+    if (d->max_batch_frames) {
+        c->starts_off = 64;
+        c->frames_off = align_up(c->starts_off + 4ull * (d->max_batch_frames + 1), ALIGN);
+        c->table_bytes = c->frames_off + sizeof(FrameDesc) * (size_t)d->max_batch_frames;
+        c->arena_bytes = d->max_batch_pixels * (3ull * c->s4 + 1) + (uint64_t)d->max_batch_frames * 4 * ALIGN;
+        for (auto& s : c->slots) {
+            if (hipHostMalloc(&s.staging, c->table_bytes, hipHostMallocDefault) != hipSuccess ||
+                hipMalloc(&s.table, c->table_bytes) != hipSuccess || hipMalloc(&s.arena, c->arena_bytes) != hipSuccess ||
+                hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+                destroy_context(c);
+                return SVGF_ERR_OUT_OF_MEMORY;
+            }
+            std::memset(s.staging, 0, c->table_bytes);
+        }
+        if (hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+            destroy_context(c);
+            return SVGF_ERR_HIP;
+        }
+    }
+    *out = c;
+    return SVGF_OK;
+}
+
+svgf_status svgf_destroy(svgf_context* c)
+{
+    if (!c) return SVGF_ERR_INVALID_ARGUMENT;
+    DeviceGuard guard(c->desc.device);
+    destroy_context(c);
+    return SVGF_OK;
+}
+
+svgf_status svgf_set_params(svgf_context* c, const svgf_params* p)
+{
+    if (!c || !p) return SVGF_ERR_INVALID_ARGUMENT;
+    svgf_status s = validate_params(*p, c->desc.mode);
+    if (s != SVGF_OK) return s;
+    DeviceGuard guard(c->desc.device);
+    drop_graphs(c);  // captured kernels hold the old parameters by value
+    c->desc.params = *p;
+    c->kp = to_kparams(*p);
+    return SVGF_OK;
+}
+
+svgf_status svgf_denoise(svgf_context* c, const svgf_frame* fr, void* hip_stream)
+{
+    if (!c || !fr) return SVGF_ERR_INVALID_ARGUMENT;
+    const uint32_t w = fr->width, h = fr->height;
+    if (w < 1 || h < 1) return SVGF_ERR_INVALID_ARGUMENT;
+    if (w > c->desc.max_width || h > c->desc.max_height) return SVGF_ERR_CAPACITY;
+    svgf_status s;
+    const bool hv = fr->history_valid != 0;
+    if ((s = check_plane(fr->radiance, fmt4(c), w)) != SVGF_OK) return s;
+    if ((s = check_plane(fr->moments_curr, fmt2(c), w)) != SVGF_OK) return s;
+    if ((s = check_plane(fr->denoised, fmt4(c), w)) != SVGF_OK) return s;
+    if (hv && (s = check_plane(fr->moments_prev, fmt2(c), w)) != SVGF_OK) return s;
+    if (c->mode == SVGF_MODE_CANONICAL) {
+        if ((s = check_plane(fr->gbuffer, SVGF_FMT_RGBA32F, w)) != SVGF_OK) return s;
+        if ((s = check_plane(fr->history_curr, fmt4(c), w)) != SVGF_OK) return s;
+        if (hv) {
+            if ((s = check_plane(fr->gbuffer_prev, SVGF_FMT_RGBA32F, w)) != SVGF_OK) return s;
+            if ((s = check_plane(fr->motion, SVGF_FMT_RG32F, w)) != SVGF_OK) return s;
+            if ((s = check_plane(fr->history_prev, fmt4(c), w)) != SVGF_OK) return s;
+        }
+    }
+    DeviceGuard guard(c->desc.device);
+    FrameDesc f{};
+    f.w = (int32_t)w;
+    f.h = (int32_t)h;
+    f.history_valid = hv ? 1 : 0;
+    f.tiles_x = (int32_t)((w + TILE_W - 1) / TILE_W);
+    f.rad = dev_plane(fr->radiance);
+    f.mom_curr = dev_plane(fr->moments_curr);
+    f.den = dev_plane(fr->denoised);
+    if (hv) f.mom_prev = dev_plane(fr->moments_prev);
+    if (c->mode == SVGF_MODE_CANONICAL) {
+        f.g = dev_plane(fr->gbuffer);
+        f.hist_curr = dev_plane(fr->history_curr);
+        if (hv) {
+            f.gprev = dev_plane(fr->gbuffer_prev);
+            f.mv = dev_plane(fr->motion);
+            f.hist_prev = dev_plane(fr->history_prev);
+        }
+    }
+    // Scratch planes are packed at this frame's width.
+    carve_scratch(static_cast<char*>(c->scratch), c->s4, w, h, f.wa, f.wb, f.ctmp, f.nlen);
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    hipEvent_t* ev = profile_events(c);
+    hipError_t e = c->mode == SVGF_MODE_REFERENCE ? launch_reference_frame(f, c->kp, c->st, st, ev)
+                                                  : launch_canonical_frame(f, c->kp, c->st, st, ev);
+    return e == hipSuccess ? SVGF_OK : SVGF_ERR_HIP;
+}
+
+svgf_status svgf_stream_open(svgf_context* c, uint32_t w, uint32_t h, uint32_t* id)
+{
+    if (!c || !id || w < 1 || h < 1) return SVGF_ERR_INVALID_ARGUMENT;
+    if (w > c->desc.max_width || h > c->desc.max_height) return SVGF_ERR_CAPACITY;
+    auto it = std::find_if(c->streams.begin(), c->streams.end(), [](const StreamSlot& s) { return !s.open; });
+    if (it == c->streams.end()) return SVGF_ERR_CAPACITY;
+    uint64_t bytes = stream_state_bytes(c->desc.mode, c->s4, c->s2, w, h);
+    if (c->desc.hbm_budget_bytes && c->budget_used + bytes > c->desc.hbm_budget_bytes) return SVGF_ERR_OVER_BUDGET;
+    DeviceGuard guard(c->desc.device);
+    void* block = nullptr;
+    if (hipMalloc(&block, bytes) != hipSuccess) return SVGF_ERR_OUT_OF_MEMORY;
+    StreamSlot& s = *it;
+    s = StreamSlot{};
+    s.open = true;
+    s.w = w;
+    s.h = h;
+    s.block = block;
+    s.bytes = bytes;
+    uint64_t px = (uint64_t)w * h;
+    char* b = static_cast<char*>(block);
+    for (int i = 0; i < 2; ++i) {
+        s.mom[i] = PlaneDev{b, (uint64_t)w * c->s2};
+        b += align_up(px * c->s2, ALIGN);
+    }
+    if (c->desc.mode == SVGF_MODE_CANONICAL) {
+        for (int i = 0; i < 2; ++i) {
+            s.hist[i] = PlaneDev{b, (uint64_t)w * c->s4};
+            b += align_up(px * c->s4, ALIGN);
+        }
+        for (int i = 0; i < 2; ++i) {
+            s.gbuf[i] = PlaneDev{b, (uint64_t)w * 16};
+            b += align_up(px * 16, ALIGN);
+        }
+    }
+    c->budget_used += bytes;
+    *id = (uint32_t)(it - c->streams.begin());
+    return SVGF_OK;
+}
+
+static StreamSlot* find_stream(svgf_context* c, uint32_t id)
+{
+    if (!c || id >= c->streams.size() || !c->streams[id].open) return nullptr;
+    return &c->streams[id];
+}
+
+svgf_status svgf_stream_close(svgf_context* c, uint32_t id)
+{
+    StreamSlot* s = find_stream(c, id);
+    if (!s) return c ? SVGF_ERR_NO_STREAM : SVGF_ERR_INVALID_ARGUMENT;
+    DeviceGuard guard(c->desc.device);
+    // Batches in flight may still read the stream's state.
+    for (auto& sl : c->slots)
+        if (sl.pending) {
+            (void)hipEventSynchronize(sl.done);
+            sl.pending = false;
+        }
+    (void)hipFree(s->block);
+    c->budget_used -= s->bytes;
+    *s = StreamSlot{};
+    return SVGF_OK;
+}
+
+svgf_status svgf_stream_reset(svgf_context* c, uint32_t id, void* hip_stream)
+{
+    (void)hip_stream;
+    StreamSlot* s = find_stream(c, id);
+    if (!s) return c ? SVGF_ERR_NO_STREAM : SVGF_ERR_INVALID_ARGUMENT;
+    // Cold history: the next frame runs with history_valid = 0 and never reads the old state
+    // (the zero-cleared images of main.cpp:1449-1455 / 1570-1603).
+    s->frames = 0;
+    return SVGF_OK;
+}
+
+svgf_status svgf_stream_state_get(svgf_context* c, uint32_t id, svgf_stream_state* out)
+{
+    StreamSlot* s = find_stream(c, id);
+    if (!s) return c ? SVGF_ERR_NO_STREAM : SVGF_ERR_INVALID_ARGUMENT;
+    if (!out) return SVGF_ERR_INVALID_ARGUMENT;
+    *out = svgf_stream_state{};
+    out->width = s->w;
+    out->height = s->h;
+    out->frames = s->frames;
+    int p = s->parity;
+    out->moments = svgf_plane{s->mom[p].ptr, s->mom[p].pitch, fmt2(c), 0};
+    if (c->desc.mode == SVGF_MODE_CANONICAL) {
+        out->history = svgf_plane{s->hist[p].ptr, s->hist[p].pitch, fmt4(c), 0};
+        out->gbuffer = svgf_plane{s->gbuf[p].ptr, s->gbuf[p].pitch, SVGF_FMT_RGBA32F, 0};
+    }
+    return SVGF_OK;
+}
+
+svgf_status svgf_budget_used(svgf_context* c, uint64_t* bytes)
+{
+    if (!c || !bytes) return SVGF_ERR_INVALID_ARGUMENT;
+    *bytes = c->budget_used;
+    return SVGF_OK;
+}
+
+svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_t n, void* hip_stream)
+{
+    if (!c || (!req && n)) return SVGF_ERR_INVALID_ARGUMENT;
+    if (n == 0) return SVGF_OK;
+    if (!c->desc.max_batch_frames) return SVGF_ERR_UNSUPPORTED;
+    if (n > c->desc.max_batch_frames) return SVGF_ERR_CAPACITY;
+    uint64_t total_px = 0, arena_need = 0;
+    uint32_t total_tiles = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        StreamSlot* s = find_stream(c, req[i].stream_id);
+        if (!s) return SVGF_ERR_NO_STREAM;
+        for (uint32_t j = 0; j < i; ++j)
+            if (req[j].stream_id == req[i].stream_id) return SVGF_ERR_INVALID_ARGUMENT;
+        svgf_status st;
+        if ((st = check_plane(req[i].radiance, fmt4(c), s->w)) != SVGF_OK) return st;
+        if ((st = check_plane(req[i].denoised, fmt4(c), s->w)) != SVGF_OK) return st;
+        if (c->mode == SVGF_MODE_CANONICAL) {
+            if ((st = check_plane(req[i].gbuffer, SVGF_FMT_RGBA32F, s->w)) != SVGF_OK) return st;
+            if (s->frames > 0 && (st = check_plane(req[i].motion, SVGF_FMT_RG32F, s->w)) != SVGF_OK) return st;
+        }
+        total_px += (uint64_t)s->w * s->h;
+        arena_need += frame_scratch_bytes(c->s4, (uint64_t)s->w * s->h);
+        total_tiles += tiles_of(s->w, s->h);
+    }
+    if (total_px > c->desc.max_batch_pixels || arena_need > c->arena_bytes) return SVGF_ERR_CAPACITY;
+
+    DeviceGuard guard(c->desc.device);
+    BatchSlot& slot = c->slots[c->next_slot];
+    c->next_slot = (c->next_slot + 1) % BATCH_SLOTS;
+    if (slot.pending) {
+        if (hipEventSynchronize(slot.done) != hipSuccess) return SVGF_ERR_HIP;
+        slot.pending = false;
+    }
+    char* host = static_cast<char*>(slot.staging);
+    uint32_t* hdr = reinterpret_cast<uint32_t*>(host);
+    uint32_t* starts = reinterpret_cast<uint32_t*>(host + c->starts_off);
+    FrameDesc* frames = reinterpret_cast<FrameDesc*>(host + c->frames_off);
+    char* arena = static_cast<char*>(slot.arena);
+    uint32_t tile = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        StreamSlot& s = c->streams[req[i].stream_id];
+        FrameDesc f{};
+        f.w = (int32_t)s.w;
+        f.h = (int32_t)s.h;
+        f.history_valid = s.frames > 0 ? 1 : 0;
+        f.tiles_x = (int32_t)((s.w + TILE_W - 1) / TILE_W);
+        const int rd = s.parity, wr = s.parity ^ 1;
+        f.rad = dev_plane(req[i].radiance);
+        f.den = dev_plane(req[i].denoised);
+        f.mom_prev = s.mom[rd];
+        f.mom_curr = s.mom[wr];
+        if (c->mode == SVGF_MODE_CANONICAL) {
+            f.g = dev_plane(req[i].gbuffer);
+            if (f.history_valid) f.mv = dev_plane(req[i].motion);
+            f.gprev = s.gbuf[rd];
+            f.gsave = s.gbuf[wr];
+            f.hist_prev = s.hist[rd];
+            f.hist_curr = s.hist[wr];
+        }
+        carve_scratch(arena, c->s4, s.w, s.h, f.wa, f.wb, f.ctmp, f.nlen);
+        arena += frame_scratch_bytes(c->s4, (uint64_t)s.w * s.h);
+        frames[i] = f;
+        starts[i] = tile;
+        tile += tiles_of(s.w, s.h);
+    }
+    starts[n] = tile;
+    hdr[0] = n;
+    hdr[1] = total_tiles;
+
+    hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+    hipError_t e = hipSuccess;
+    if (c->desc.batch_flags & (SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES)) {
+        e = enqueue_batch(c, slot, stream, profile_events(c));
+    } else {
+        if (!slot.exec) {
+            e = hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                hipError_t le = enqueue_batch(c, slot, c->capture_stream);
+                e = hipStreamEndCapture(c->capture_stream, &slot.graph);
+                if (le != hipSuccess) e = le;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&slot.exec, slot.graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                if (slot.graph) (void)hipGraphDestroy(slot.graph);
+                slot.graph = nullptr;
+                slot.exec = nullptr;
+                return SVGF_ERR_HIP;
+            }
+        }
+        e = hipGraphLaunch(slot.exec, stream);
+    }
+    if (e != hipSuccess) return SVGF_ERR_HIP;
+    if (hipEventRecord(slot.done, stream) != hipSuccess) return SVGF_ERR_HIP;
+    slot.pending = true;
+    for (uint32_t i = 0; i < n; ++i) {
+        StreamSlot& s = c->streams[req[i].stream_id];
+        s.frames++;
+        s.parity ^= 1;
+    }
+    return SVGF_OK;
+}
+
+svgf_status svgf_stage_times(svgf_context* c, float* ms, uint32_t capacity, uint32_t* count)
+{
+    if (!c || !count || (!ms && capacity)) return SVGF_ERR_INVALID_ARGUMENT;
+    if (!(c->desc.batch_flags & SVGF_PROFILE_STAGES) || c->ev_stages == 0) return SVGF_ERR_UNSUPPORTED;
+    DeviceGuard guard(c->desc.device);
+    *count = (uint32_t)c->ev_stages;
+    if (hipEventSynchronize(c->ev[c->ev_stages]) != hipSuccess) return SVGF_ERR_HIP;
+    for (int i = 0; i < c->ev_stages && (uint32_t)i < capacity; ++i)
+        if (hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]) != hipSuccess) return SVGF_ERR_HIP;
+    return SVGF_OK;
+}
+
+}  // extern "C"

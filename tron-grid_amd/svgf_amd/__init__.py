@@ -1,0 +1,356 @@
+"""Python host binding of the MI355X SVGF library (C ABI of include/svgf.h via ctypes).
+
+The compute path is the HIP library tron-grid_amd/lib/libsvgf_mi355x.so; this module only
+marshals descriptors. PyTorch tensors serve as device memory and streams. There is no CPU
+fallback: loading fails loudly when the library has not been built.
+
+The single-frame call mirrors TronGrid's SVGF pass (src/main.cpp:907-947 recording the four
+dispatches of src/svgf_denoise.slang:152-275): same images, same parameters, same ping-pong of
+moments / history, with cold history = zero images (src/main.cpp:1449-1455).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+import re
+import subprocess
+
+import numpy as np
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent.parent  # tron-grid_amd/
+REPO_DIR = PKG_DIR.parent
+LIB_PATH = PKG_DIR / "lib" / "libsvgf_mi355x.so"
+HEADER_PATH = REPO_DIR / "include" / "svgf.h"
+
+MODE_REFERENCE = 0
+MODE_CANONICAL = 1
+STORAGE_FP32 = 0
+STORAGE_FP16 = 1
+FMT_RGBA32F, FMT_RGBA16F, FMT_RG32F, FMT_RG16F = 1, 2, 3, 4
+FLAG_COMPENSATED = 0x1
+BATCH_NO_GRAPH = 0x1
+PROFILE_STAGES = 0x2
+
+STATUS = {
+    0: "ok", 1: "invalid argument", 2: "unsupported", 3: "out of device memory", 4: "over HBM budget",
+    5: "no such stream", 6: "over context capacity", 7: "HIP runtime error", 8: "no gfx950 device", 9: "timeout",
+}
+
+
+class SvgfError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({status})")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint32),
+        ("phi_luma", ctypes.c_float),
+        ("phi_normal", ctypes.c_float),
+        ("phi_depth", ctypes.c_float),
+        ("alpha_color", ctypes.c_float),
+        ("alpha_moments", ctypes.c_float),
+        ("variance_eps", ctypes.c_float),
+        ("depth_eps", ctypes.c_float),
+        ("depth_reject", ctypes.c_float),
+        ("normal_reject", ctypes.c_float),
+        ("history_cap", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_uint32),
+        ("storage", ctypes.c_uint32),
+        ("params", Params),
+        ("max_width", ctypes.c_uint32),
+        ("max_height", ctypes.c_uint32),
+        ("max_streams", ctypes.c_uint32),
+        ("max_batch_frames", ctypes.c_uint32),
+        ("max_batch_pixels", ctypes.c_uint64),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("device", ctypes.c_int32),
+        ("batch_flags", ctypes.c_uint32),
+    ]
+
+
+class Plane(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("pitch_bytes", ctypes.c_uint64), ("format", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("history_valid", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)] + [(n, Plane) for n in (
+                    "radiance", "gbuffer", "gbuffer_prev", "motion", "history_prev", "history_curr",
+                    "moments_prev", "moments_curr", "denoised")]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("stream_id", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("radiance", Plane),
+                ("gbuffer", Plane), ("motion", Plane), ("denoised", Plane)]
+
+
+class StreamState(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("frames", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("history", Plane), ("moments", Plane), ("gbuffer", Plane)]
+
+
+class SynthDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+        ("stream", ctypes.c_uint64), ("frame", ctypes.c_uint32), ("noise", ctypes.c_uint32),
+        ("pan_px", ctypes.c_float), ("disocclusion", ctypes.c_float), ("threads", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compile the HIP library in-tree (hipcc --offload-arch=gfx950)."""
+    args = ["make", "-s", "-C", str(PKG_DIR), f"-j{min(8, os.cpu_count() or 1)}"]
+    if force:
+        subprocess.run(["make", "-s", "-C", str(PKG_DIR), "clean"], check=True)
+    subprocess.run(args, check=True)
+
+
+def header_functions() -> list[str]:
+    """Every function include/svgf.h declares."""
+    text = HEADER_PATH.read_text()
+    return re.findall(r"^SVGF_API\s+[\w\s\*]+?\b(svgf_\w+)\s*\(", text, flags=re.M)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        L.svgf_abi_version.restype = ctypes.c_uint32
+        L.svgf_status_str.restype = ctypes.c_char_p
+        L.svgf_status_str.argtypes = [ctypes.c_int]
+        for name, args in {
+            "svgf_default_params": [ctypes.c_uint32, ctypes.c_uint32, P(Params)],
+            "svgf_query_bytes": [P(Desc), ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_uint64), P(ctypes.c_uint64)],
+            "svgf_create": [P(Desc), P(vp)],
+            "svgf_destroy": [vp],
+            "svgf_set_params": [vp, P(Params)],
+            "svgf_denoise": [vp, P(Frame), vp],
+            "svgf_stream_open": [vp, ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_uint32)],
+            "svgf_stream_close": [vp, ctypes.c_uint32],
+            "svgf_stream_reset": [vp, ctypes.c_uint32, vp],
+            "svgf_stream_state_get": [vp, ctypes.c_uint32, P(StreamState)],
+            "svgf_budget_used": [vp, P(ctypes.c_uint64)],
+            "svgf_denoise_batch": [vp, P(Request), ctypes.c_uint32, vp],
+            "svgf_synth_frame": [P(SynthDesc), vp, vp, vp],
+            "svgf_stage_times": [vp, P(ctypes.c_float), ctypes.c_uint32, P(ctypes.c_uint32)],
+        }.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise SvgfError(status, what)
+
+
+def default_params(mode: int, storage: int = STORAGE_FP32) -> Params:
+    p = Params()
+    check(lib().svgf_default_params(mode, storage, ctypes.byref(p)), "svgf_default_params")
+    return p
+
+
+# ---------------------------------------------------------------------------------------------
+# synthetic inputs (host, no GPU)
+# ---------------------------------------------------------------------------------------------
+def synth_frame(w, h, *, seed=0, stream=0, frame=0, noise=0, pan_px=0.5, disocclusion=0.0, threads=0,
+                radiance=True, gbuffer=True, motion=True):
+    """Seeded synthetic inputs: dict of float32 arrays radiance (h,w,4), gbuffer (h,w,4),
+    motion (h,w,2)."""
+    d = SynthDesc(w, h, seed, stream, frame, noise, pan_px, disocclusion, threads, 0)
+    out = {}
+    if radiance:
+        out["radiance"] = np.empty((h, w, 4), np.float32)
+    if gbuffer:
+        out["gbuffer"] = np.empty((h, w, 4), np.float32)
+    if motion:
+        out["motion"] = np.empty((h, w, 2), np.float32)
+
+    def ptr(k):
+        return out[k].ctypes.data if k in out else None
+
+    check(lib().svgf_synth_frame(ctypes.byref(d), ptr("radiance"), ptr("gbuffer"), ptr("motion")),
+          "svgf_synth_frame")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# device planes (torch as allocator)
+# ---------------------------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def plane_of(t, fmt: int) -> Plane:
+    """Plane descriptor of a contiguous (h, w, c) torch tensor on the GPU."""
+    if t is None:
+        return Plane()
+    assert t.is_cuda and t.is_contiguous(), "planes are contiguous device tensors"
+    return Plane(t.data_ptr(), t.stride(0) * t.element_size(), fmt, 0)
+
+
+def storage_dtype(storage: int):
+    torch = _torch()
+    return torch.float32 if storage == STORAGE_FP32 else torch.float16
+
+
+def fmt4(storage):
+    return FMT_RGBA32F if storage == STORAGE_FP32 else FMT_RGBA16F
+
+
+def fmt2(storage):
+    return FMT_RG32F if storage == STORAGE_FP32 else FMT_RG16F
+
+
+class Context:
+    """One svgf_context (replaces the SVGF pipeline + descriptor sets of src/main.cpp:2288-2446)."""
+
+    def __init__(self, mode=MODE_REFERENCE, storage=STORAGE_FP32, params=None, max_width=1920, max_height=1080,
+                 max_streams=0, max_batch_frames=0, max_batch_pixels=0, hbm_budget_bytes=0, device=0,
+                 batch_flags=0):
+        self.mode, self.storage = mode, storage
+        self.params = params if params is not None else default_params(mode, storage)
+        self.desc = Desc(mode, storage, self.params, max_width, max_height, max_streams, max_batch_frames,
+                         max_batch_pixels, hbm_budget_bytes, device, batch_flags)
+        h = ctypes.c_void_p()
+        check(lib().svgf_create(ctypes.byref(self.desc), ctypes.byref(h)), "svgf_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            check(lib().svgf_destroy(self.handle), "svgf_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p: Params):
+        check(lib().svgf_set_params(self.handle, ctypes.byref(p)), "svgf_set_params")
+        self.params = p
+
+    def denoise(self, *, radiance, denoised, moments_curr, moments_prev=None, gbuffer=None, gbuffer_prev=None,
+                motion=None, history_prev=None, history_curr=None, history_valid=True, stream=None):
+        """Enqueue one frame on `stream` (a torch.cuda.Stream or raw handle; default: current)."""
+        st = self.storage
+        f = Frame()
+        h, w = radiance.shape[:2]
+        f.width, f.height, f.history_valid = w, h, int(bool(history_valid))
+        f.radiance = plane_of(radiance, fmt4(st))
+        f.denoised = plane_of(denoised, fmt4(st))
+        f.moments_curr = plane_of(moments_curr, fmt2(st))
+        f.moments_prev = plane_of(moments_prev, fmt2(st))
+        f.gbuffer = plane_of(gbuffer, FMT_RGBA32F)
+        f.gbuffer_prev = plane_of(gbuffer_prev, FMT_RGBA32F)
+        f.motion = plane_of(motion, FMT_RG32F)
+        f.history_prev = plane_of(history_prev, fmt4(st))
+        f.history_curr = plane_of(history_curr, fmt4(st))
+        check(lib().svgf_denoise(self.handle, ctypes.byref(f), _stream_handle(stream)), "svgf_denoise")
+
+    # -- resident streams / batched form --
+    def stream_open(self, w, h) -> int:
+        sid = ctypes.c_uint32()
+        check(lib().svgf_stream_open(self.handle, w, h, ctypes.byref(sid)), "svgf_stream_open")
+        return sid.value
+
+    def stream_close(self, sid):
+        check(lib().svgf_stream_close(self.handle, sid), "svgf_stream_close")
+
+    def stream_reset(self, sid, stream=None):
+        check(lib().svgf_stream_reset(self.handle, sid, _stream_handle(stream)), "svgf_stream_reset")
+
+    def stream_state(self, sid) -> StreamState:
+        s = StreamState()
+        check(lib().svgf_stream_state_get(self.handle, sid, ctypes.byref(s)), "svgf_stream_state_get")
+        return s
+
+    def budget_used(self) -> int:
+        b = ctypes.c_uint64()
+        check(lib().svgf_budget_used(self.handle, ctypes.byref(b)), "svgf_budget_used")
+        return b.value
+
+    def stage_times(self):
+        """GPU milliseconds of each stage of the last call (needs PROFILE_STAGES)."""
+        buf = (ctypes.c_float * 16)()
+        n = ctypes.c_uint32()
+        check(lib().svgf_stage_times(self.handle, buf, 16, ctypes.byref(n)), "svgf_stage_times")
+        return [buf[i] for i in range(n.value)]
+
+    def denoise_batch(self, items, stream=None):
+        """items: list of dicts {stream_id, radiance, denoised, gbuffer?, motion?} (tensors)."""
+        st = self.storage
+        arr = (Request * max(1, len(items)))()
+        for i, it in enumerate(items):
+            r = arr[i]
+            r.stream_id = it["stream_id"]
+            r.radiance = plane_of(it["radiance"], fmt4(st))
+            r.denoised = plane_of(it["denoised"], fmt4(st))
+            r.gbuffer = plane_of(it.get("gbuffer"), FMT_RGBA32F)
+            r.motion = plane_of(it.get("motion"), FMT_RG32F)
+        check(lib().svgf_denoise_batch(self.handle, arr, len(items), _stream_handle(stream)), "svgf_denoise_batch")
+
+
+def _stream_handle(stream):
+    if stream is None:
+        torch = _torch()
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def device_tensor_from_plane(plane: Plane, h: int, w: int, channels: int, dtype):
+    """Copy a library-owned plane (e.g. resident stream state) into a new torch tensor."""
+    torch = _torch()
+    out = torch.empty((h, w, channels), dtype=dtype, device="cuda")
+    elem = out.element_size() * channels
+    import ctypes as _c
+    hip = _hip()
+    rc = hip.hipMemcpy2D(_c.c_void_p(out.data_ptr()), _c.c_size_t(w * elem), _c.c_void_p(plane.ptr),
+                         _c.c_size_t(plane.pitch_bytes), _c.c_size_t(w * elem), _c.c_size_t(h), 3)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy2D failed: {rc}")
+    return out
+
+
+_hip_lib = None
+
+
+def _hip():
+    global _hip_lib
+    if _hip_lib is None:
+        _hip_lib = ctypes.CDLL("libamdhip64.so")
+        _hip_lib.hipMemcpy2D.restype = ctypes.c_int
+    return _hip_lib
