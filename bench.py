@@ -31,17 +31,89 @@ HBM_PEAK_GBPS = 8000.0
 
 # Algorithmic bytes per pixel, per stage, stage-structured rule: each stage reads each distinct
 # plane once and writes each output plane once at its storage width (extra taps count 0).
-#   temporal : radiance + G + motion + G_prev + history + moments read; colour/var + moments +
-#              length (u8) written (+ the G copy kept for the next frame in the batched form)
-#   variance : colour/var + G + length read; wavelet written
+#   temporal : radiance + G + motion + G_prev + history + moments read; wavelet W0 (colour,
+#              sigma) + moments + length (u8) written (+ the G copy kept for the next frame in
+#              the batched form)
+#   variance : length read; pixels with young history (n < 4) also read their 3x3 window and
+#              patch W0's sigma in place (data dependent, not counted)
 #   a-trous  : wavelet + G read; wavelet (or output) written; iteration 0 also reads the length
 #              and writes the colour history
 def stage_bytes(storage_bytes4, storage_bytes2, iterations, batched):
     s4, s2 = storage_bytes4, storage_bytes2
     temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (s4 + s2 + 1) + (16 if batched else 0)
-    variance = (s4 + 16 + 1) + s4
+    variance = 1
     atrous = [(s4 + 16) + s4 + ((1 + s4) if k == 0 else 0) for k in range(iterations)]
     return [temporal, variance] + atrous
+
+
+def _rel(a, b, floor=1e-3):
+    import numpy as np
+
+    r = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) / np.maximum(np.abs(b), floor)
+    return float(r.max()), float(r.mean())
+
+
+def precision_report(frames: int, threads: int, scale: int) -> dict:
+    """Error of the GPU forms against the FP64 oracle over whole sequences:
+    plain FP32 on 1920x1080 (exponential noise, 0.02 m/frame pan, 10% disocclusion) and the
+    FP16 precision form on 3840x2160 (lognormal sigma 1.5). `scale` divides both sizes."""
+    import numpy as np
+    import torch
+
+    import svgf_amd as S
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    dev = torch.device("cuda:0")
+    out = {}
+    # camera translation 0.02 m/frame seen at 10 m through a 60-degree vertical field of view
+    for name, (w, h), storage, noise in (("plain_fp32_1080p", (1920, 1080), S.STORAGE_FP32, 0),
+                                         ("precision_fp16_4k", (3840, 2160), S.STORAGE_FP16, 1)):
+        w, h = w // scale, h // scale
+        pan = 0.02 * (h / (2.0 * np.tan(np.pi / 6))) / 10.0
+        dt = torch.float32 if storage == S.STORAGE_FP32 else torch.float16
+        p = S.default_params(S.MODE_CANONICAL, storage)
+        op = O.canonical_params(p.iterations, compensated=bool(p.flags & S.FLAG_COMPENSATED))
+        hist = [torch.zeros((h, w, 4), dtype=dt, device=dev) for _ in range(2)]
+        mom = [torch.zeros((h, w, 2), dtype=dt, device=dev) for _ in range(2)]
+        gb = [None, None]
+        ohist = omom = ogp = None
+        worst = {k: [0.0, 0.0] for k in ("denoised", "history", "moments_mean")}
+        t_gpu = 0.0
+        with S.Context(S.MODE_CANONICAL, storage, params=p, max_width=w, max_height=h) as ctx:
+            for f in range(frames):
+                d = S.synth_frame(w, h, seed=f, frame=f, noise=noise, pan_px=pan, disocclusion=0.1)
+                rad = d["radiance"]
+                if storage == S.STORAGE_FP16:
+                    rad = rad.astype(np.float16).astype(np.float32)  # the input the FP16 form receives
+                cur, prev = f % 2, (f + 1) % 2
+                gb[cur] = torch.from_numpy(d["gbuffer"]).to(dev)
+                den = torch.empty((h, w, 4), dtype=dt, device=dev)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ctx.denoise(radiance=torch.from_numpy(rad).to(dev).to(dt), gbuffer=gb[cur],
+                            gbuffer_prev=gb[prev] if f else None, motion=torch.from_numpy(d["motion"]).to(dev),
+                            history_prev=hist[prev] if f else None, history_curr=hist[cur],
+                            moments_prev=mom[prev] if f else None, moments_curr=mom[cur], denoised=den,
+                            history_valid=f > 0)
+                torch.cuda.synchronize()
+                t_gpu += time.perf_counter() - t0
+                o = O.c_frame_f64(rad, d["gbuffer"], ogp, d["motion"], ohist, omom, op, history_valid=f > 0,
+                                  threads=threads)
+                ohist, omom, ogp = o["history"], o["moments"], d["gbuffer"]
+                for key, g, r in (("denoised", den[..., :3], o["denoised"][..., :3]),
+                                  ("history", hist[cur][..., :3], o["history"][..., :3]),
+                                  ("moments_mean", mom[cur][..., 0], o["moments"][..., 0])):
+                    mx, mean = _rel(g.float().cpu().numpy(), r)
+                    worst[key][0] = max(worst[key][0], mx)
+                    worst[key][1] = max(worst[key][1], mean)
+        out[name] = {"size": f"{w}x{h}", "frames": frames, "storage": "fp32" if storage == S.STORAGE_FP32 else "fp16",
+                     "compensated_moments": bool(p.flags & S.FLAG_COMPENSATED),
+                     "max_rel_err": {k: v[0] for k, v in worst.items()},
+                     "mean_rel_err_worst_frame": {k: v[1] for k, v in worst.items()},
+                     "rel_err_floor": 1e-3}
+    return out
 
 
 def main():
@@ -50,10 +122,22 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--frames", type=int, default=64, help="frames per batch")
-    ap.add_argument("--distinct", type=int, default=4, help="distinct input frames per stream (cycled)")
+    ap.add_argument("--distinct", type=int, default=16, help="distinct input frames per stream (cycled)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="serving simulation length (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=4000.0, help="mean Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-streams", type=int, default=256)
+    ap.add_argument("--report", choices=["precision"], default=None,
+                    help="precision: error of the GPU forms vs the FP64 oracle on 16-frame 1080p / 4K sequences")
+    ap.add_argument("--report-frames", type=int, default=16)
+    ap.add_argument("--report-scale", type=int, default=1)
+    ap.add_argument("--report-threads", type=int, default=16)
     args = ap.parse_args()
+
+    if args.report == "precision":
+        print(json.dumps({"precision": precision_report(args.report_frames, args.report_threads, args.report_scale)}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
@@ -80,14 +164,22 @@ def main():
 
     # ---- inputs: ND frames per stream, generated on the host, resident in HBM ----
     t0 = time.time()
-    host = [[S.synth_frame(w, h, seed=1000 * i + f, stream=i, frame=f, pan_px=0.5, disocclusion=0.05)
-             for f in range(ND)] for i, (w, h) in enumerate(shapes)]
-    gen_s = time.time() - t0
-    rad16 = [[torch.from_numpy(d["radiance"]).to(dev).half() for d in fr] for fr in host]
-    rad32 = [[torch.from_numpy(d["radiance"]).to(dev) for d in fr] for fr in host]
-    gbuf = [[torch.from_numpy(d["gbuffer"]).to(dev) for d in fr] for fr in host]
-    mvec = [[torch.from_numpy(d["motion"]).to(dev) for d in fr] for fr in host]
+    host = [[None] * 2 for _ in shapes]  # frames 0 and 1 stay on the host for the CPU baseline
+    rad16 = [[None] * ND for _ in shapes]
+    rad32 = [[None] * ND for _ in shapes]
+    gbuf = [[None] * ND for _ in shapes]
+    mvec = [[None] * ND for _ in shapes]
+    for i, (w, h) in enumerate(shapes):
+        for f in range(ND):
+            d = S.synth_frame(w, h, seed=1000 * i + f, stream=i, frame=f, pan_px=0.5, disocclusion=0.05)
+            rad16[i][f] = torch.from_numpy(d["radiance"]).to(dev).half()
+            rad32[i][f] = torch.from_numpy(d["radiance"]).to(dev)
+            gbuf[i][f] = torch.from_numpy(d["gbuffer"]).to(dev)
+            mvec[i][f] = torch.from_numpy(d["motion"]).to(dev)
+            if f < 2:
+                host[i][f] = d
     torch.cuda.synchronize()
+    gen_s = time.time() - t0
 
     # ---- batched FP16 canonical form ----
     p16 = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
@@ -187,6 +279,29 @@ def main():
     bytes_px_16 = sum(sb16)
     bytes_px_32 = sum(stage_bytes(16, 8, p32.iterations, batched=False))
 
+    # ---- serving form: Poisson arrivals from 256 agent streams, continuous batching ----
+    serving = None
+    if args.serve_seconds > 0:
+        classes = RESOLUTIONS
+        class_inputs = []
+        for c, (w, h) in enumerate(classes):
+            ins = []
+            for f in range(2):
+                d = S.synth_frame(w, h, seed=7000 + 10 * c + f, stream=10000 + c, frame=f, pan_px=0.5, disocclusion=0.05)
+                ins.append({"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                            "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                            "motion": torch.from_numpy(d["motion"]).to(dev)})
+            class_inputs.append(ins)
+        torch.cuda.synchronize()
+        budget = 64 << 30
+        sctx = S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p16, max_width=1280, max_height=720,
+                         max_streams=args.serve_streams + 64, max_batch_frames=args.frames,
+                         max_batch_pixels=max(batch_px, 1280 * 720 * 8), hbm_budget_bytes=budget)
+        with S.Server(sctx, lanes=2) as srv:
+            serving = srv.simulate(streams=args.serve_streams, classes=classes, class_inputs=class_inputs,
+                                   rate_hz=args.serve_rate, duration_s=args.serve_seconds, churn_per_s=0.05, seed=11)
+        sctx.close()
+
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
     cpu = None
     if not args.no_cpu_baseline:
@@ -232,11 +347,20 @@ def main():
             "iterations": p16.iterations,
             "params": {k: round(v, 6) if isinstance(v, float) else v for k, v in p16.as_dict().items()},
             "bytes_per_px": bytes_px_16,
-            "plane_layout": "radiance/history/wavelet rgba16f, moments (mean, sigma) rg16f, G normal.xyz+depth "
-                            "rgba32f, motion rg32f, history length u8, G copy rgba32f",
+            "plane_layout": "radiance/history/wavelet (colour, sigma) rgba16f, moments (mean, sigma) rg16f, "
+                            "G normal.xyz+depth rgba32f, motion rg32f, history length u8, G copy rgba32f",
         },
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "what": "GPU time of one 64-frame batch"},
+        "serving": None if serving is None else {
+            "what": f"{args.serve_streams} agent streams, Poisson {args.serve_rate:g} req/s, 5%/s churn (cold "
+                    f"history), 64 GB HBM budget, 2 in-flight batches, {args.serve_seconds:g} s",
+            "p50_ms": round(serving["latency_p50_ms"], 4), "p99_ms": round(serving["latency_p99_ms"], 4),
+            "max_ms": round(serving["latency_max_ms"], 4), "mpx_per_s": round(serving["mpx_per_s"], 2),
+            "offered_mpx_per_s": round(serving["offered_mpx_per_s"], 2),
+            "requests": int(serving["completed"]), "rejected": int(serving["rejected"]),
+            "mean_batch_frames": round(serving["mean_batch_frames"], 2),
+            "streams_opened": int(serving["streams_opened"])},
         "plain_fp32": {"value": round(plain_mpx, 2), "unit": "Mpx/s", "bytes_per_px": bytes_px_32,
                        "ratio_batched_over_plain": round(batched_mpx / plain_mpx, 3)},
         "effective_GBps": round(batched_mpx * 1e6 * bytes_px_16 / 1e9, 1),

@@ -206,6 +206,80 @@ SVGF_API svgf_status svgf_denoise_batch(svgf_context* ctx, const svgf_request* r
  * temporal, variance, a-trous 0..N-1. *count receives the number of stages. */
 SVGF_API svgf_status svgf_stage_times(svgf_context* ctx, float* ms, uint32_t capacity, uint32_t* count);
 
+/* --- serving form: continuous batching over resident streams -----------------------------
+ * A server owns a context (created with max_streams / max_batch_* set) and a scheduler thread.
+ * Requests are queued by svgf_server_submit from any thread. Whenever one of `lanes` in-flight
+ * slots is free, the scheduler packs queued requests (FIFO, at most one per stream, a stream
+ * whose previous frame is still on the GPU waits) into one svgf_denoise_batch on that lane's
+ * HIP stream, so batches overlap on the device and new arrivals join the next batch. Results
+ * of a request do not depend on what it was batched with. While a server runs, use the
+ * svgf_server_* stream calls instead of svgf_stream_* on its context. */
+typedef struct svgf_server svgf_server;
+
+typedef struct svgf_server_desc {
+    uint32_t lanes;            /* batches in flight at once (1..2) */
+    uint32_t max_batch_frames; /* <= context max_batch_frames */
+    uint64_t max_batch_pixels; /* <= context max_batch_pixels */
+    uint32_t max_wait_us;      /* hold a partial batch this long for more requests (0 = none) */
+    uint32_t reserved;
+} svgf_server_desc;
+
+typedef struct svgf_completion {
+    uint64_t tag;          /* caller's tag from svgf_server_submit */
+    uint32_t stream_id;
+    int32_t status;        /* svgf_status of the batch that carried it */
+    uint64_t submit_ns;    /* steady-clock times: queued, launched, observed complete */
+    uint64_t launch_ns;
+    uint64_t done_ns;
+    uint32_t batch_frames; /* frames in the batch that carried it */
+    uint32_t reserved;
+} svgf_completion;
+
+SVGF_API svgf_status svgf_server_create(svgf_context* ctx, const svgf_server_desc* desc, svgf_server** out);
+SVGF_API svgf_status svgf_server_destroy(svgf_server* srv);
+SVGF_API svgf_status svgf_server_stream_open(svgf_server* srv, uint32_t width, uint32_t height, uint32_t* stream_id);
+SVGF_API svgf_status svgf_server_stream_close(svgf_server* srv, uint32_t stream_id);
+/* Thread-safe. The request's planes must stay valid until its completion is polled. */
+SVGF_API svgf_status svgf_server_submit(svgf_server* srv, const svgf_request* req, uint64_t tag);
+/* Returns up to `capacity` completions, waiting up to wait_us for the first one. */
+SVGF_API svgf_status svgf_server_poll(svgf_server* srv, svgf_completion* out, uint32_t capacity,
+                                      uint32_t* count, uint32_t wait_us);
+/* Blocks until every submitted request has completed (completions stay pollable). */
+SVGF_API svgf_status svgf_server_drain(svgf_server* srv);
+
+/* Open-loop load test of a server (the serving benchmark): Poisson arrivals at rate_hz from
+ * `streams` agent streams whose resolutions are drawn uniformly from the class list, streams
+ * replaced (closed, reopened cold with a new resolution) at churn_per_s of the population per
+ * second. Inputs come from `inputs_per_class` device frames per class (radiance, gbuffer,
+ * motion of class_inputs[class * inputs_per_class + k]; their `denoised` field is ignored);
+ * outputs go to per-stream buffers the simulation allocates. */
+typedef struct svgf_sim_desc {
+    uint32_t streams;
+    uint32_t n_classes;
+    const uint32_t* class_width;
+    const uint32_t* class_height;
+    const svgf_request* class_inputs;
+    uint32_t inputs_per_class;
+    uint32_t reserved;
+    double rate_hz;
+    double duration_s;
+    double churn_per_s;
+    uint64_t seed;
+} svgf_sim_desc;
+
+typedef struct svgf_sim_result {
+    uint64_t submitted, completed, rejected; /* rejected: admission over budget */
+    uint64_t batches, streams_opened, streams_closed;
+    uint64_t pixels;             /* completed pixels */
+    double wall_s;               /* first arrival to last completion */
+    double mpx_per_s;
+    double latency_p50_ms, latency_p99_ms, latency_max_ms, latency_mean_ms;
+    double mean_batch_frames;
+    double offered_mpx_per_s;
+} svgf_sim_result;
+
+SVGF_API svgf_status svgf_server_simulate(svgf_server* srv, const svgf_sim_desc* desc, svgf_sim_result* out);
+
 /* --- seeded synthetic G-buffer sequences (host memory; no GPU needed) -------------------- */
 typedef struct svgf_synth_desc {
     uint32_t width;

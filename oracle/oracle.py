@@ -74,6 +74,7 @@ def lib():
         L.oracle_r_frame_f64.argtypes = [ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, dp, dp, dp, dp, dp, dp]
         L.oracle_c_frame_f64.argtypes = [ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, fp, fp, fp, fp,
                                          dp, dp, dp, dp, dp, dp, dp]
+        L.oracle_c_frame_f64_mt.argtypes = L.oracle_c_frame_f64.argtypes + [ctypes.c_int]
         L.oracle_f32_to_f16.argtypes = [ctypes.c_float]
         L.oracle_f32_to_f16.restype = ctypes.c_uint16
         L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
@@ -142,7 +143,7 @@ def r_frame_f64(lighting, moments_prev, params=None, history_valid=True):
 
 
 def c_frame_f64(radiance, gbuffer, gbuffer_prev, motion, history_prev, moments_prev, params=None,
-                history_valid=True, debug=False):
+                history_valid=True, debug=False, threads=1):
     """One CANONICAL frame in double. Returns dict with history, moments, denoised (and
     temporal / variance stage outputs when debug)."""
     p = params or canonical_params()
@@ -159,10 +160,10 @@ def c_frame_f64(radiance, gbuffer, gbuffer_prev, motion, history_prev, moments_p
         out["temporal"] = np.zeros((h, w, 4))
         out["variance"] = np.zeros((h, w, 4))
     gp = _f32(gbuffer_prev) if gbuffer_prev is not None else np.zeros((h, w, 4), np.float32)
-    rc = lib().oracle_c_frame_f64(
+    rc = lib().oracle_c_frame_f64_mt(
         w, h, ctypes.byref(p), int(bool(history_valid)), _fp(radiance), _fp(_f32(gbuffer)), _fp(gp),
         _fp(_f32(motion)), _dp(hist_prev), _dp(mom_prev), _dp(out["history"]), _dp(out["moments"]),
-        _dp(out["denoised"]), _dp(out.get("temporal")), _dp(out.get("variance")))
+        _dp(out["denoised"]), _dp(out.get("temporal")), _dp(out.get("variance")), int(threads))
     if rc:
         raise ValueError(f"oracle_c_frame_f64 failed: {rc}")
     return out

@@ -356,34 +356,29 @@ static void depth_gradient(const float* g, int w, int h, int x, int y, double* g
     *gy = 0.5 * ((double)g[4 * ((size_t)yd * w + x) + 3] - (double)g[4 * ((size_t)yu * w + x) + 3]);
 }
 
-int oracle_c_frame_f64(int w, int h, const oracle_params* p, int history_valid,
-                       const float* radiance, const float* gbuffer, const float* gbuffer_prev,
-                       const float* motion, const double* history_prev, const double* moments_prev,
-                       double* history_curr, double* moments_curr, double* denoised,
-                       double* dbg_temporal, double* dbg_variance)
-{
-    if (w <= 0 || h <= 0 || !p || p->iterations < 1 || p->iterations > 16) return 1;
-    if (!radiance || !history_curr || !moments_curr || !denoised) return 1;
-    const int as_ref = (p->flags & ORACLE_FLAG_AS_REFERENCE) != 0;
-    const int comp = (p->flags & ORACLE_FLAG_COMPENSATED) != 0 && !as_ref;
-    if (!as_ref && (!gbuffer || !motion)) return 1;
-    if (history_valid && (!moments_prev || (!as_ref && (!history_prev || !gbuffer_prev)))) return 1;
-    const size_t np = (size_t)w * h;
-    double* C = (double*)malloc(np * 4 * sizeof(double));
-    int* N = (int*)malloc(np * sizeof(int));
-    double* WA = (double*)malloc(np * 4 * sizeof(double));
-    double* WB = (double*)malloc(np * 4 * sizeof(double));
-    if (!C || !N || !WA || !WB) {
-        free(C);
-        free(N);
-        free(WA);
-        free(WB);
-        return 2;
-    }
-    const double alpha_m = p->alpha_moments, alpha_c = p->alpha_color;
+typedef struct c_job {
+    int w, h, history_valid, as_ref, comp, k;
+    const oracle_params* p;
+    const float *radiance, *gbuffer, *gbuffer_prev, *motion;
+    const double *history_prev, *moments_prev;
+    double *history_curr, *moments_curr, *denoised;
+    double *C, *WA, *WB;
+    const double* src;
+    double* dst;
+    int* N;
+    int y0, y1;
+} c_job;
 
-    /* ---- temporal ---- */
-    for (int y = 0; y < h; ++y) {
+static void c_temporal_rows(const c_job* J)
+{
+    const int w = J->w, h = J->h, history_valid = J->history_valid, as_ref = J->as_ref, comp = J->comp;
+    const oracle_params* p = J->p;
+    const float *radiance = J->radiance, *gbuffer = J->gbuffer, *gbuffer_prev = J->gbuffer_prev, *motion = J->motion;
+    const double *history_prev = J->history_prev, *moments_prev = J->moments_prev;
+    double *moments_curr = J->moments_curr, *C = J->C;
+    int* N = J->N;
+    const double alpha_m = p->alpha_moments, alpha_c = p->alpha_color;
+    for (int y = J->y0; y < J->y1; ++y) {
         for (int x = 0; x < w; ++x) {
             size_t i = (size_t)y * w + x;
             const float* R = radiance + 4 * i;
@@ -492,17 +487,17 @@ int oracle_c_frame_f64(int w, int h, const oracle_params* p, int history_valid,
             N[i] = n_new;
         }
     }
-    if (dbg_temporal) {
-        for (size_t i = 0; i < np; ++i) {
-            dbg_temporal[4 * i + 0] = C[4 * i + 0];
-            dbg_temporal[4 * i + 1] = C[4 * i + 1];
-            dbg_temporal[4 * i + 2] = C[4 * i + 2];
-            dbg_temporal[4 * i + 3] = as_ref ? C[4 * i + 3] : (double)N[i];
-        }
-    }
+}
 
-    /* ---- variance ---- */
-    for (int y = 0; y < h; ++y) {
+static void c_variance_rows(const c_job* J)
+{
+    const int w = J->w, h = J->h, as_ref = J->as_ref;
+    const oracle_params* p = J->p;
+    const float* gbuffer = J->gbuffer;
+    const double* C = J->C;
+    double* WA = J->WA;
+    const int* N = J->N;
+    for (int y = J->y0; y < J->y1; ++y) {
         for (int x = 0; x < w; ++x) {
             size_t i = (size_t)y * w + x;
             double var = C[4 * i + 3];
@@ -538,90 +533,212 @@ int oracle_c_frame_f64(int w, int h, const oracle_params* p, int history_valid,
             WA[4 * i + 3] = var;
         }
     }
+}
+
+static void c_atrous_rows(const c_job* J)
+{
+    const int w = J->w, h = J->h, as_ref = J->as_ref, k = J->k;
+    const oracle_params* p = J->p;
+    const float* gbuffer = J->gbuffer;
+    const double* src = J->src;
+    double *dst = J->dst, *history_curr = J->history_curr, *denoised = J->denoised;
+    const int* N = J->N;
+    const int s = 1 << k;
+    const int fin = k == (int)p->iterations - 1;
+    for (int y = J->y0; y < J->y1; ++y) {
+        for (int x = 0; x < w; ++x) {
+            size_t i = (size_t)y * w + x;
+            const double* c = src + 4 * i;
+            double lc = luma_d(c);
+            double sigma;
+            double gx = 0.0, gy = 0.0;
+            const float* Gp = NULL;
+            if (as_ref) {
+                sigma = fmax((double)p->phi_luma * c[3], (double)p->variance_eps);
+            } else {
+                double gv = 0.0;
+                static const double K3[3] = {0.25, 0.5, 0.25};
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int qx = clampi(x + dx, 0, w - 1), qy = clampi(y + dy, 0, h - 1);
+                        gv += K3[dx + 1] * K3[dy + 1] * src[4 * ((size_t)qy * w + qx) + 3];
+                    }
+                sigma = (double)p->phi_luma * sqrt(fmax(gv, 0.0) + (double)p->variance_eps);
+                Gp = gbuffer + 4 * i;
+                depth_gradient(gbuffer, w, h, x, y, &gx, &gy);
+            }
+            double s0 = 0, s1 = 0, s2 = 0, sw = 0, sv = 0;
+            for (int dy = -2; dy <= 2; ++dy) {
+                for (int dx = -2; dx <= 2; ++dx) {
+                    int qx = clampi(x + dx * s, 0, w - 1), qy = clampi(y + dy * s, 0, h - 1);
+                    size_t q = (size_t)qy * w + qx;
+                    const double* v = src + 4 * q;
+                    double arg = fabs(luma_d(v) - lc) / sigma;
+                    double wn = 1.0;
+                    if (!as_ref) {
+                        double az;
+                        wn = depth_normal_weight(p, Gp, gbuffer + 4 * q, gx, gy, (double)dx * s, (double)dy * s, &az);
+                        arg += az;
+                    }
+                    double wt = (double)KERNEL5[dx + 2] * (double)KERNEL5[dy + 2] * exp(-arg) * wn;
+                    s0 += wt * v[0];
+                    s1 += wt * v[1];
+                    s2 += wt * v[2];
+                    sw += wt;
+                    sv += wt * wt * v[3];
+                }
+            }
+            double* o = dst + 4 * i;
+            if (sw > 1e-6) {
+                o[0] = s0 / sw;
+                o[1] = s1 / sw;
+                o[2] = s2 / sw;
+                o[3] = as_ref ? c[3] : sv / (sw * sw);
+            } else {
+                o[0] = c[0];
+                o[1] = c[1];
+                o[2] = c[2];
+                o[3] = c[3];
+            }
+            if (k == 0 && !as_ref) {
+                history_curr[4 * i + 0] = o[0];
+                history_curr[4 * i + 1] = o[1];
+                history_curr[4 * i + 2] = o[2];
+                history_curr[4 * i + 3] = (double)N[i];
+            }
+            if (fin) {
+                denoised[4 * i + 0] = o[0];
+                denoised[4 * i + 1] = o[1];
+                denoised[4 * i + 2] = o[2];
+                denoised[4 * i + 3] = 1.0;
+            }
+        }
+    }
+}
+
+typedef void (*c_rows_fn)(const c_job*);
+typedef struct c_task {
+    c_rows_fn fn;
+    c_job job;
+} c_task;
+
+static void* c_task_entry(void* arg)
+{
+    c_task* t = (c_task*)arg;
+    t->fn(&t->job);
+    return NULL;
+}
+
+/* Runs fn over all rows, split statically over `threads` pthreads (rows are independent
+ * within a stage). */
+static void c_run_rows(c_rows_fn fn, const c_job* base, int threads)
+{
+    if (threads <= 1) {
+        c_job j = *base;
+        j.y0 = 0;
+        j.y1 = base->h;
+        fn(&j);
+        return;
+    }
+    c_task tasks[256];
+    pthread_t tid[256];
+    for (int t = 0; t < threads; ++t) {
+        tasks[t].fn = fn;
+        tasks[t].job = *base;
+        tasks[t].job.y0 = (int)((int64_t)base->h * t / threads);
+        tasks[t].job.y1 = (int)((int64_t)base->h * (t + 1) / threads);
+        pthread_create(&tid[t], NULL, c_task_entry, &tasks[t]);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+}
+
+int oracle_c_frame_f64_mt(int w, int h, const oracle_params* p, int history_valid,
+                          const float* radiance, const float* gbuffer, const float* gbuffer_prev,
+                          const float* motion, const double* history_prev, const double* moments_prev,
+                          double* history_curr, double* moments_curr, double* denoised,
+                          double* dbg_temporal, double* dbg_variance, int threads)
+{
+    if (w <= 0 || h <= 0 || !p || p->iterations < 1 || p->iterations > 16) return 1;
+    if (!radiance || !history_curr || !moments_curr || !denoised) return 1;
+    const int as_ref = (p->flags & ORACLE_FLAG_AS_REFERENCE) != 0;
+    const int comp = (p->flags & ORACLE_FLAG_COMPENSATED) != 0 && !as_ref;
+    if (!as_ref && (!gbuffer || !motion)) return 1;
+    if (history_valid && (!moments_prev || (!as_ref && (!history_prev || !gbuffer_prev)))) return 1;
+    if (threads < 1) threads = 1;
+    if (threads > h) threads = h;
+    if (threads > 256) threads = 256;
+    const size_t np = (size_t)w * h;
+    double* C = (double*)malloc(np * 4 * sizeof(double));
+    int* N = (int*)malloc(np * sizeof(int));
+    double* WA = (double*)malloc(np * 4 * sizeof(double));
+    double* WB = (double*)malloc(np * 4 * sizeof(double));
+    if (!C || !N || !WA || !WB) {
+        free(C);
+        free(N);
+        free(WA);
+        free(WB);
+        return 2;
+    }
+    c_job J;
+    memset(&J, 0, sizeof J);
+    J.w = w;
+    J.h = h;
+    J.history_valid = history_valid;
+    J.as_ref = as_ref;
+    J.comp = comp;
+    J.p = p;
+    J.radiance = radiance;
+    J.gbuffer = gbuffer;
+    J.gbuffer_prev = gbuffer_prev;
+    J.motion = motion;
+    J.history_prev = history_prev;
+    J.moments_prev = moments_prev;
+    J.history_curr = history_curr;
+    J.moments_curr = moments_curr;
+    J.denoised = denoised;
+    J.C = C;
+    J.WA = WA;
+    J.WB = WB;
+    J.N = N;
+
+    /* ---- temporal ---- */
+    c_run_rows(c_temporal_rows, &J, threads);
+    if (dbg_temporal) {
+        for (size_t i = 0; i < np; ++i) {
+            dbg_temporal[4 * i + 0] = C[4 * i + 0];
+            dbg_temporal[4 * i + 1] = C[4 * i + 1];
+            dbg_temporal[4 * i + 2] = C[4 * i + 2];
+            dbg_temporal[4 * i + 3] = as_ref ? C[4 * i + 3] : (double)N[i];
+        }
+    }
+
+    /* ---- variance ---- */
+    c_run_rows(c_variance_rows, &J, threads);
     if (dbg_variance) memcpy(dbg_variance, WA, np * 4 * sizeof(double));
 
     /* ---- a-trous ---- */
-    const double* src = WA;
-    double* dst = WB;
+    J.src = WA;
+    J.dst = WB;
     for (int k = 0; k < (int)p->iterations; ++k) {
-        int s = 1 << k;
-        int fin = k == (int)p->iterations - 1;
-        for (int y = 0; y < h; ++y) {
-            for (int x = 0; x < w; ++x) {
-                size_t i = (size_t)y * w + x;
-                const double* c = src + 4 * i;
-                double lc = luma_d(c);
-                double sigma;
-                double gx = 0.0, gy = 0.0;
-                const float* Gp = NULL;
-                if (as_ref) {
-                    sigma = fmax((double)p->phi_luma * c[3], (double)p->variance_eps);
-                } else {
-                    double gv = 0.0;
-                    static const double K3[3] = {0.25, 0.5, 0.25};
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            int qx = clampi(x + dx, 0, w - 1), qy = clampi(y + dy, 0, h - 1);
-                            gv += K3[dx + 1] * K3[dy + 1] * src[4 * ((size_t)qy * w + qx) + 3];
-                        }
-                    sigma = (double)p->phi_luma * sqrt(fmax(gv, 0.0) + (double)p->variance_eps);
-                    Gp = gbuffer + 4 * i;
-                    depth_gradient(gbuffer, w, h, x, y, &gx, &gy);
-                }
-                double s0 = 0, s1 = 0, s2 = 0, sw = 0, sv = 0;
-                for (int dy = -2; dy <= 2; ++dy) {
-                    for (int dx = -2; dx <= 2; ++dx) {
-                        int qx = clampi(x + dx * s, 0, w - 1), qy = clampi(y + dy * s, 0, h - 1);
-                        size_t q = (size_t)qy * w + qx;
-                        const double* v = src + 4 * q;
-                        double arg = fabs(luma_d(v) - lc) / sigma;
-                        double wn = 1.0;
-                        if (!as_ref) {
-                            double az;
-                            wn = depth_normal_weight(p, Gp, gbuffer + 4 * q, gx, gy, (double)dx * s, (double)dy * s, &az);
-                            arg += az;
-                        }
-                        double wt = (double)KERNEL5[dx + 2] * (double)KERNEL5[dy + 2] * exp(-arg) * wn;
-                        s0 += wt * v[0];
-                        s1 += wt * v[1];
-                        s2 += wt * v[2];
-                        sw += wt;
-                        sv += wt * wt * v[3];
-                    }
-                }
-                double* o = dst + 4 * i;
-                if (sw > 1e-6) {
-                    o[0] = s0 / sw;
-                    o[1] = s1 / sw;
-                    o[2] = s2 / sw;
-                    o[3] = as_ref ? c[3] : sv / (sw * sw);
-                } else {
-                    o[0] = c[0];
-                    o[1] = c[1];
-                    o[2] = c[2];
-                    o[3] = c[3];
-                }
-                if (k == 0 && !as_ref) {
-                    history_curr[4 * i + 0] = o[0];
-                    history_curr[4 * i + 1] = o[1];
-                    history_curr[4 * i + 2] = o[2];
-                    history_curr[4 * i + 3] = (double)N[i];
-                }
-                if (fin) {
-                    denoised[4 * i + 0] = o[0];
-                    denoised[4 * i + 1] = o[1];
-                    denoised[4 * i + 2] = o[2];
-                    denoised[4 * i + 3] = 1.0;
-                }
-            }
-        }
-        const double* t = src;
-        src = dst;
-        dst = (double*)t;
+        J.k = k;
+        c_run_rows(c_atrous_rows, &J, threads);
+        const double* t = J.src;
+        J.src = J.dst;
+        J.dst = (double*)t;
     }
     free(C);
     free(N);
     free(WA);
     free(WB);
     return 0;
+}
+
+int oracle_c_frame_f64(int w, int h, const oracle_params* p, int history_valid,
+                       const float* radiance, const float* gbuffer, const float* gbuffer_prev,
+                       const float* motion, const double* history_prev, const double* moments_prev,
+                       double* history_curr, double* moments_curr, double* denoised,
+                       double* dbg_temporal, double* dbg_variance)
+{
+    return oracle_c_frame_f64_mt(w, h, p, history_valid, radiance, gbuffer, gbuffer_prev, motion, history_prev,
+                                 moments_prev, history_curr, moments_curr, denoised, dbg_temporal, dbg_variance, 1);
 }

@@ -74,6 +74,12 @@ int oracle_c_frame_f64(int w, int h, const oracle_params* p, int history_valid,
                        const float* motion, const double* history_prev, const double* moments_prev,
                        double* history_curr, double* moments_curr, double* denoised,
                        double* dbg_temporal, double* dbg_variance);
+/* Same, each stage split by rows over `threads` pthreads (results identical). */
+int oracle_c_frame_f64_mt(int w, int h, const oracle_params* p, int history_valid,
+                          const float* radiance, const float* gbuffer, const float* gbuffer_prev,
+                          const float* motion, const double* history_prev, const double* moments_prev,
+                          double* history_curr, double* moments_curr, double* denoised,
+                          double* dbg_temporal, double* dbg_variance, int threads);
 
 #ifdef __cplusplus
 }

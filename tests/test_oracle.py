@@ -386,6 +386,22 @@ def test_canonical_flat_edges_reduce_to_b3_blur_of_variance():
     np.testing.assert_allclose(out["denoised"][..., :3], col, rtol=1e-6)
 
 
+def test_canonical_threads_do_not_change_result():
+    rng = np.random.default_rng(18)
+    h, w = 23, 31
+    L = rand_img(rng, h, w)
+    g = _flat_g(h, w)
+    g[..., 3] = 3 + rng.random((h, w)).astype(np.float32)
+    mv = (rng.random((h, w, 2)) * 2 - 1).astype(np.float32)
+    hist = rng.random((h, w, 4)) * 3
+    hist[..., 3] = 3
+    mom = rng.random((h, w, 2))
+    a = O.c_frame_f64(L, g, g, mv, hist, mom, O.canonical_params(), debug=True, threads=1)
+    b = O.c_frame_f64(L, g, g, mv, hist, mom, O.canonical_params(), debug=True, threads=6)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
 def test_canonical_spatial_variance_is_window_variance():
     """Young history (n < 4) with depth / normal stops off: the variance is the plain
     3x3 (clamped) variance of the temporal luminance."""

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <new>
 #include <vector>
 
@@ -50,6 +51,9 @@ struct BatchSlot {
     hipGraphExec_t exec = nullptr;
     hipEvent_t done = nullptr;
     bool pending = false;
+    std::vector<uint32_t> ids;  // streams the pending batch touches
+    std::vector<hipGraphNode_t> knodes;      // captured stage kernels, launch order
+    std::vector<hipKernelNodeParams> kparams;
 };
 
 struct DeviceGuard {
@@ -74,12 +78,17 @@ struct svgf_context {
     Storage st = Storage::F32;
     int mode = SVGF_MODE_REFERENCE;
     uint32_t s4 = 16, s2 = 8;  // storage bytes of colour-like and moment planes
+    uint32_t w4 = 16;          // bytes of a wavelet-scratch pixel (FP32 in CANONICAL mode)
 
     void* scratch = nullptr;
-    PlaneDev wa{}, wb{}, ctmp{}, nlen{};
+    PlaneDev wa{}, wb{}, nlen{};
 
     std::vector<StreamSlot> streams;
     uint64_t budget_used = 0;
+    // Freed stream-state blocks kept for reuse: closing and reopening streams (churn) then
+    // needs no hipFree, which would synchronise the whole device.
+    std::multimap<uint64_t, void*> pool;
+    uint64_t pool_bytes = 0;
 
     BatchSlot slots[BATCH_SLOTS];
     int next_slot = 0;
@@ -152,28 +161,24 @@ uint64_t stream_state_bytes(uint32_t mode, uint32_t s4, uint32_t s2, uint32_t w,
     return b;
 }
 
-// Scratch of one frame in a batch or of the single-frame form: wa, wb, ctmp (storage x4) and
-// the uint8 history length.
+// Scratch of one frame in a batch or of the single-frame form: the wavelet ping-pong
+// (storage x4) and the uint8 history length.
 uint64_t frame_scratch_bytes(uint32_t s4, uint64_t px)
 {
-    return 3 * align_up(px * s4, ALIGN) + align_up(px, ALIGN);
+    return 2 * align_up(px * s4, ALIGN) + align_up(px, ALIGN);
 }
 
-void carve_scratch(char* base, uint32_t s4, uint32_t w, uint32_t h, PlaneDev& wa, PlaneDev& wb, PlaneDev& ctmp,
-                   PlaneDev& nlen)
+void carve_scratch(char* base, uint32_t s4, uint32_t w, uint32_t h, PlaneDev& wa, PlaneDev& wb, PlaneDev& nlen)
 {
     uint64_t px = (uint64_t)w * h;
     size_t p4 = align_up(px * s4, ALIGN);
     wa = PlaneDev{base, (uint64_t)w * s4};
     wb = PlaneDev{base + p4, (uint64_t)w * s4};
-    ctmp = PlaneDev{base + 2 * p4, (uint64_t)w * s4};
-    nlen = PlaneDev{base + 3 * p4, (uint64_t)w};
+    nlen = PlaneDev{base + 2 * p4, (uint64_t)w};
 }
 
-uint32_t tiles_of(uint32_t w, uint32_t h)
-{
-    return ((w + TILE_W - 1) / TILE_W) * ((h + TILE_H - 1) / TILE_H);
-}
+// Kernels address planes with 32-bit offsets through buffer resources.
+bool plane_fits(const svgf_plane& pl, uint32_t h) { return pl.pitch_bytes * (uint64_t)h < (1ull << 31); }
 
 void free_slot(BatchSlot& s)
 {
@@ -206,6 +211,7 @@ void destroy_context(svgf_context* c)
     for (auto& s : c->slots) free_slot(s);
     for (auto& s : c->streams)
         if (s.block) (void)hipFree(s.block);
+    for (auto& kv : c->pool) (void)hipFree(kv.second);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
     for (auto& e : c->ev)
@@ -222,15 +228,64 @@ hipEvent_t* profile_events(svgf_context* c)
     return c->ev;
 }
 
-hipError_t enqueue_batch(svgf_context* c, BatchSlot& s, hipStream_t stream, hipEvent_t* ev = nullptr)
+hipError_t enqueue_batch(svgf_context* c, BatchSlot& s, const uint32_t* grids, hipStream_t stream,
+                         hipEvent_t* ev = nullptr)
 {
     hipError_t e = hipMemcpyAsync(s.table, s.staging, c->table_bytes, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return e;
     char* t = static_cast<char*>(s.table);
     BatchTable bt{reinterpret_cast<const uint32_t*>(t), reinterpret_cast<const uint32_t*>(t + c->starts_off),
                   reinterpret_cast<const FrameDesc*>(t + c->frames_off)};
-    if (c->mode == SVGF_MODE_REFERENCE) return launch_reference_batch(bt, c->kp, c->st, c->grid_blocks, stream, ev);
-    return launch_canonical_batch(bt, c->kp, c->st, c->grid_blocks, stream, ev);
+    if (c->mode == SVGF_MODE_REFERENCE) return launch_reference_batch(bt, c->kp, c->st, grids, stream, ev);
+    return launch_canonical_batch(bt, c->kp, c->st, grids, stream, ev);
+}
+
+// Captures table copy + stage launches of a slot once; later batches only change the grid
+// size of each kernel node (one block per tile), so the graph is never re-instantiated.
+hipError_t capture_slot(svgf_context* c, BatchSlot& s, const uint32_t* grids)
+{
+    hipError_t e = hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = enqueue_batch(c, s, grids, c->capture_stream);
+    e = hipStreamEndCapture(c->capture_stream, &s.graph);
+    if (le != hipSuccess) e = le;
+    if (e == hipSuccess) e = hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0);
+    // kernel nodes in launch order: follow the dependency chain from the root (the copy)
+    s.knodes.clear();
+    s.kparams.clear();
+    hipGraphNode_t node = nullptr;
+    size_t n = 1;
+    if (e == hipSuccess) e = hipGraphGetRootNodes(s.graph, &node, &n);
+    while (e == hipSuccess && n == 1 && node) {
+        hipGraphNodeType type;
+        e = hipGraphNodeGetType(node, &type);
+        if (e == hipSuccess && type == hipGraphNodeTypeKernel) {
+            hipKernelNodeParams kp{};
+            e = hipGraphKernelNodeGetParams(node, &kp);
+            s.knodes.push_back(node);
+            s.kparams.push_back(kp);
+        }
+        hipGraphNode_t next = nullptr;
+        size_t nd = 0;
+        if (e == hipSuccess) e = hipGraphNodeGetDependentNodes(node, nullptr, &nd);
+        if (e != hipSuccess || nd == 0) break;
+        if (nd != 1) {
+            e = hipErrorUnknown;
+            break;
+        }
+        e = hipGraphNodeGetDependentNodes(node, &next, &nd);
+        node = next;
+    }
+    if (e == hipSuccess && (int)s.knodes.size() != stage_count(c)) e = hipErrorUnknown;
+    if (e != hipSuccess) {
+        if (s.exec) (void)hipGraphExecDestroy(s.exec);
+        if (s.graph) (void)hipGraphDestroy(s.graph);
+        s.exec = nullptr;
+        s.graph = nullptr;
+        s.knodes.clear();
+        s.kparams.clear();
+    }
+    return e;
 }
 
 }  // namespace
@@ -291,15 +346,13 @@ svgf_status svgf_query_bytes(const svgf_desc* d, uint32_t w, uint32_t h, uint64_
 {
     if (!d || d->mode > SVGF_MODE_CANONICAL || d->storage > SVGF_STORAGE_FP16) return SVGF_ERR_INVALID_ARGUMENT;
     uint32_t s4 = d->storage == SVGF_STORAGE_FP32 ? 16 : 8, s2 = d->storage == SVGF_STORAGE_FP32 ? 8 : 4;
+    const uint32_t w4 = d->mode == SVGF_MODE_CANONICAL ? 16 : s4;
     if (context_bytes) {
-        uint64_t b = frame_scratch_bytes(s4, (uint64_t)d->max_width * d->max_height);
+        uint64_t b = frame_scratch_bytes(w4, (uint64_t)d->max_width * d->max_height);
         if (d->max_batch_frames) {
-            uint64_t max_tiles = d->max_batch_pixels / (TILE_W * TILE_H) +
-                                 (uint64_t)d->max_batch_frames * (d->max_width / TILE_W + d->max_height / TILE_H + 2);
-            (void)max_tiles;
-            uint64_t table = align_up(64 + 4ull * (d->max_batch_frames + 1), ALIGN) +
+            uint64_t table = align_up(64 + 4ull * MAX_STAGES * STARTS_ROW, ALIGN) +
                              sizeof(FrameDesc) * (uint64_t)d->max_batch_frames;
-            uint64_t arena = d->max_batch_pixels * (3ull * s4 + 1) + (uint64_t)d->max_batch_frames * 4 * ALIGN;
+            uint64_t arena = d->max_batch_pixels * (2ull * w4 + 1) + (uint64_t)d->max_batch_frames * 3 * ALIGN;
             b += BATCH_SLOTS * (table + arena);
         }
         *context_bytes = b;
@@ -315,6 +368,8 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
     if (d->mode > SVGF_MODE_CANONICAL || d->storage > SVGF_STORAGE_FP16) return SVGF_ERR_INVALID_ARGUMENT;
     if (d->max_width < 1 || d->max_height < 1 || d->max_width > 16384 || d->max_height > 16384)
         return SVGF_ERR_INVALID_ARGUMENT;
+    // every plane, the float4 G-buffer included, must stay addressable with 32-bit offsets
+    if ((uint64_t)d->max_width * d->max_height * 16 >= (1ull << 31)) return SVGF_ERR_CAPACITY;
     if (d->max_batch_frames > MAX_BATCH_FRAMES) return SVGF_ERR_INVALID_ARGUMENT;
     if (d->max_batch_frames && d->max_batch_pixels < 1) return SVGF_ERR_INVALID_ARGUMENT;
     if (d->batch_flags & ~(SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES)) return SVGF_ERR_INVALID_ARGUMENT;
@@ -335,16 +390,17 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
     c->st = d->storage == SVGF_STORAGE_FP32 ? Storage::F32 : Storage::F16;
     c->s4 = c->st == Storage::F32 ? 16 : 8;
     c->s2 = c->st == Storage::F32 ? 8 : 4;
+    c->w4 = c->mode == SVGF_MODE_CANONICAL ? 16 : c->s4;
     c->kp = to_kparams(d->params);
     c->grid_blocks = std::max(1, prop.multiProcessorCount) * 8;
 
-    uint64_t sb = frame_scratch_bytes(c->s4, (uint64_t)d->max_width * d->max_height);
+    uint64_t sb = frame_scratch_bytes(c->w4,(uint64_t)d->max_width * d->max_height);
     if (hipMalloc(&c->scratch, sb) != hipSuccess) {
         c->scratch = nullptr;
         destroy_context(c);
         return SVGF_ERR_OUT_OF_MEMORY;
     }
-    carve_scratch(static_cast<char*>(c->scratch), c->s4, d->max_width, d->max_height, c->wa, c->wb, c->ctmp, c->nlen);
+    carve_scratch(static_cast<char*>(c->scratch), c->w4,d->max_width, d->max_height, c->wa, c->wb, c->nlen);
     if (d->batch_flags & SVGF_PROFILE_STAGES) {
         for (auto& e : c->ev)
             if (hipEventCreate(&e) != hipSuccess) {
@@ -357,9 +413,9 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
     if (d->max_streams) c->streams.resize(d->max_streams);
     if (d->max_batch_frames) {
         c->starts_off = 64;
-        c->frames_off = align_up(c->starts_off + 4ull * (d->max_batch_frames + 1), ALIGN);
+        c->frames_off = align_up(c->starts_off + 4ull * MAX_STAGES * STARTS_ROW, ALIGN);
         c->table_bytes = c->frames_off + sizeof(FrameDesc) * (size_t)d->max_batch_frames;
-        c->arena_bytes = d->max_batch_pixels * (3ull * c->s4 + 1) + (uint64_t)d->max_batch_frames * 4 * ALIGN;
+        c->arena_bytes = d->max_batch_pixels * (2ull * c->w4 + 1) + (uint64_t)d->max_batch_frames * 3 * ALIGN;
         for (auto& s : c->slots) {
             if (hipHostMalloc(&s.staging, c->table_bytes, hipHostMallocDefault) != hipSuccess ||
                 hipMalloc(&s.table, c->table_bytes) != hipSuccess || hipMalloc(&s.arena, c->arena_bytes) != hipSuccess ||
@@ -419,12 +475,14 @@ svgf_status svgf_denoise(svgf_context* c, const svgf_frame* fr, void* hip_stream
             if ((s = check_plane(fr->history_prev, fmt4(c), w)) != SVGF_OK) return s;
         }
     }
+    for (const svgf_plane* pl : {&fr->radiance, &fr->gbuffer, &fr->gbuffer_prev, &fr->motion, &fr->history_prev,
+                                 &fr->history_curr, &fr->moments_prev, &fr->moments_curr, &fr->denoised})
+        if (pl->ptr && !plane_fits(*pl, h)) return SVGF_ERR_CAPACITY;
     DeviceGuard guard(c->desc.device);
     FrameDesc f{};
     f.w = (int32_t)w;
     f.h = (int32_t)h;
     f.history_valid = hv ? 1 : 0;
-    f.tiles_x = (int32_t)((w + TILE_W - 1) / TILE_W);
     f.rad = dev_plane(fr->radiance);
     f.mom_curr = dev_plane(fr->moments_curr);
     f.den = dev_plane(fr->denoised);
@@ -439,7 +497,7 @@ svgf_status svgf_denoise(svgf_context* c, const svgf_frame* fr, void* hip_stream
         }
     }
     // Scratch planes are packed at this frame's width.
-    carve_scratch(static_cast<char*>(c->scratch), c->s4, w, h, f.wa, f.wb, f.ctmp, f.nlen);
+    carve_scratch(static_cast<char*>(c->scratch), c->w4,w, h, f.wa, f.wb, f.nlen);
     hipStream_t st = static_cast<hipStream_t>(hip_stream);
     hipEvent_t* ev = profile_events(c);
     hipError_t e = c->mode == SVGF_MODE_REFERENCE ? launch_reference_frame(f, c->kp, c->st, st, ev)
@@ -454,10 +512,27 @@ svgf_status svgf_stream_open(svgf_context* c, uint32_t w, uint32_t h, uint32_t* 
     auto it = std::find_if(c->streams.begin(), c->streams.end(), [](const StreamSlot& s) { return !s.open; });
     if (it == c->streams.end()) return SVGF_ERR_CAPACITY;
     uint64_t bytes = stream_state_bytes(c->desc.mode, c->s4, c->s2, w, h);
-    if (c->desc.hbm_budget_bytes && c->budget_used + bytes > c->desc.hbm_budget_bytes) return SVGF_ERR_OVER_BUDGET;
+    const uint64_t budget = c->desc.hbm_budget_bytes;
+    if (budget && c->budget_used + bytes > budget) return SVGF_ERR_OVER_BUDGET;
     DeviceGuard guard(c->desc.device);
     void* block = nullptr;
-    if (hipMalloc(&block, bytes) != hipSuccess) return SVGF_ERR_OUT_OF_MEMORY;
+    auto it_pool = c->pool.lower_bound(bytes);
+    if (it_pool != c->pool.end() && it_pool->first <= 2 * bytes &&
+        (!budget || c->budget_used + it_pool->first <= budget)) {
+        block = it_pool->second;
+        bytes = it_pool->first;
+        c->pool_bytes -= it_pool->first;
+        c->pool.erase(it_pool);
+    } else {
+        // keep open streams + cached blocks within the budget
+        while (budget && !c->pool.empty() && c->budget_used + c->pool_bytes + bytes > budget) {
+            auto big = std::prev(c->pool.end());
+            (void)hipFree(big->second);
+            c->pool_bytes -= big->first;
+            c->pool.erase(big);
+        }
+        if (hipMalloc(&block, bytes) != hipSuccess) return SVGF_ERR_OUT_OF_MEMORY;
+    }
     StreamSlot& s = *it;
     s = StreamSlot{};
     s.open = true;
@@ -497,13 +572,14 @@ svgf_status svgf_stream_close(svgf_context* c, uint32_t id)
     StreamSlot* s = find_stream(c, id);
     if (!s) return c ? SVGF_ERR_NO_STREAM : SVGF_ERR_INVALID_ARGUMENT;
     DeviceGuard guard(c->desc.device);
-    // Batches in flight may still read the stream's state.
+    // Batches in flight that touch this stream may still read its state.
     for (auto& sl : c->slots)
-        if (sl.pending) {
+        if (sl.pending && std::find(sl.ids.begin(), sl.ids.end(), id) != sl.ids.end()) {
             (void)hipEventSynchronize(sl.done);
             sl.pending = false;
         }
-    (void)hipFree(s->block);
+    c->pool.emplace(s->bytes, s->block);
+    c->pool_bytes += s->bytes;
     c->budget_used -= s->bytes;
     *s = StreamSlot{};
     return SVGF_OK;
@@ -552,7 +628,6 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
     if (!c->desc.max_batch_frames) return SVGF_ERR_UNSUPPORTED;
     if (n > c->desc.max_batch_frames) return SVGF_ERR_CAPACITY;
     uint64_t total_px = 0, arena_need = 0;
-    uint32_t total_tiles = 0;
     for (uint32_t i = 0; i < n; ++i) {
         StreamSlot* s = find_stream(c, req[i].stream_id);
         if (!s) return SVGF_ERR_NO_STREAM;
@@ -564,10 +639,12 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
         if (c->mode == SVGF_MODE_CANONICAL) {
             if ((st = check_plane(req[i].gbuffer, SVGF_FMT_RGBA32F, s->w)) != SVGF_OK) return st;
             if (s->frames > 0 && (st = check_plane(req[i].motion, SVGF_FMT_RG32F, s->w)) != SVGF_OK) return st;
+            if (!plane_fits(req[i].gbuffer, s->h) || (s->frames > 0 && !plane_fits(req[i].motion, s->h)))
+                return SVGF_ERR_CAPACITY;
         }
+        if (!plane_fits(req[i].radiance, s->h) || !plane_fits(req[i].denoised, s->h)) return SVGF_ERR_CAPACITY;
         total_px += (uint64_t)s->w * s->h;
-        arena_need += frame_scratch_bytes(c->s4, (uint64_t)s->w * s->h);
-        total_tiles += tiles_of(s->w, s->h);
+        arena_need += frame_scratch_bytes(c->w4,(uint64_t)s->w * s->h);
     }
     if (total_px > c->desc.max_batch_pixels || arena_need > c->arena_bytes) return SVGF_ERR_CAPACITY;
 
@@ -583,14 +660,12 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
     uint32_t* starts = reinterpret_cast<uint32_t*>(host + c->starts_off);
     FrameDesc* frames = reinterpret_cast<FrameDesc*>(host + c->frames_off);
     char* arena = static_cast<char*>(slot.arena);
-    uint32_t tile = 0;
     for (uint32_t i = 0; i < n; ++i) {
         StreamSlot& s = c->streams[req[i].stream_id];
         FrameDesc f{};
         f.w = (int32_t)s.w;
         f.h = (int32_t)s.h;
         f.history_valid = s.frames > 0 ? 1 : 0;
-        f.tiles_x = (int32_t)((s.w + TILE_W - 1) / TILE_W);
         const int rd = s.parity, wr = s.parity ^ 1;
         f.rad = dev_plane(req[i].radiance);
         f.den = dev_plane(req[i].denoised);
@@ -604,41 +679,48 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
             f.hist_prev = s.hist[rd];
             f.hist_curr = s.hist[wr];
         }
-        carve_scratch(arena, c->s4, s.w, s.h, f.wa, f.wb, f.ctmp, f.nlen);
-        arena += frame_scratch_bytes(c->s4, (uint64_t)s.w * s.h);
+        carve_scratch(arena, c->w4, s.w, s.h, f.wa, f.wb, f.nlen);
+        arena += frame_scratch_bytes(c->w4,(uint64_t)s.w * s.h);
         frames[i] = f;
-        starts[i] = tile;
-        tile += tiles_of(s.w, s.h);
     }
-    starts[n] = tile;
+    // per stage: first tile of each frame (stage strides differ, so do the tile counts)
+    const int canonical = c->mode == SVGF_MODE_CANONICAL;
+    const int nstages = stage_count(c);
+    uint32_t grids[MAX_STAGES];
+    for (int st = 0; st < nstages; ++st) {
+        uint32_t* row = starts + (size_t)st * STARTS_ROW;
+        uint32_t tile = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            row[i] = tile;
+            tile += tiles_at((uint32_t)frames[i].w, (uint32_t)frames[i].h, stage_log2s(canonical, st));
+        }
+        row[n] = tile;
+        grids[st] = tile;
+    }
     hdr[0] = n;
-    hdr[1] = total_tiles;
 
     hipStream_t stream = static_cast<hipStream_t>(hip_stream);
     hipError_t e = hipSuccess;
     if (c->desc.batch_flags & (SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES)) {
-        e = enqueue_batch(c, slot, stream, profile_events(c));
+        e = enqueue_batch(c, slot, grids, stream, profile_events(c));
     } else {
         if (!slot.exec) {
-            e = hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal);
-            if (e == hipSuccess) {
-                hipError_t le = enqueue_batch(c, slot, c->capture_stream);
-                e = hipStreamEndCapture(c->capture_stream, &slot.graph);
-                if (le != hipSuccess) e = le;
+            if (capture_slot(c, slot, grids) != hipSuccess) return SVGF_ERR_HIP;
+        } else {
+            for (int st = 0; st < nstages && e == hipSuccess; ++st) {
+                hipKernelNodeParams kp = slot.kparams[st];
+                kp.gridDim = dim3(grids[st], 1, 1);
+                e = hipGraphExecKernelNodeSetParams(slot.exec, slot.knodes[st], &kp);
             }
-            if (e == hipSuccess) e = hipGraphInstantiate(&slot.exec, slot.graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                if (slot.graph) (void)hipGraphDestroy(slot.graph);
-                slot.graph = nullptr;
-                slot.exec = nullptr;
-                return SVGF_ERR_HIP;
-            }
+            if (e != hipSuccess) return SVGF_ERR_HIP;
         }
         e = hipGraphLaunch(slot.exec, stream);
     }
     if (e != hipSuccess) return SVGF_ERR_HIP;
     if (hipEventRecord(slot.done, stream) != hipSuccess) return SVGF_ERR_HIP;
     slot.pending = true;
+    slot.ids.clear();
+    for (uint32_t i = 0; i < n; ++i) slot.ids.push_back(req[i].stream_id);
     for (uint32_t i = 0; i < n; ++i) {
         StreamSlot& s = c->streams[req[i].stream_id];
         s.frames++;
@@ -660,3 +742,10 @@ svgf_status svgf_stage_times(svgf_context* c, float* ms, uint32_t capacity, uint
 }
 
 }  // extern "C"
+
+namespace svgf {
+uint32_t context_storage(const svgf_context* c) { return c->desc.storage; }
+uint32_t context_mode(const svgf_context* c) { return c->desc.mode; }
+int context_device(const svgf_context* c) { return c->desc.device; }
+const svgf_desc& context_desc(const svgf_context* c) { return c->desc; }
+}  // namespace svgf

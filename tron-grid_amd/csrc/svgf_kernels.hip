@@ -1,14 +1,19 @@
 // svgf_kernels.hip — SVGF stages as gfx950 kernels.
 //
-// Every stage is a per-pixel __device__ function; the single-frame kernels (2D grid over
-// one frame's 64x4 tiles) and the batched kernels (grid-stride loop over a ragged tile table
-// spanning many frames) call the same function, so a frame gives bit-identical results
-// alone and inside a batch.
+// Every stage is a block-level __device__ function over one 16x16 lattice tile (see
+// svgf_internal.hpp). The single-frame kernels launch one block per tile of one frame; the
+// batched kernels grid-stride over the tiles of every frame of a batch. Both call the same
+// function, so a frame gives bit-identical results alone and inside any batch.
+//
+// Memory: planes are read and written through raw buffer resources (SGPR descriptor + 32-bit
+// offset): no 64-bit address arithmetic per access, and an access past the end of a plane
+// reads 0 / writes nothing instead of faulting. A-trous taps come from LDS: each block stages
+// its tile plus a 2-point lattice halo once (converted to float, luminance precomputed), so
+// the 25 taps of every output cost only LDS reads.
 //
 // REFERENCE stages follow src/svgf_denoise.slang:152-275 operation by operation (no
 // contraction where the shader's rounding matters: luminance, lerp, variance). CANONICAL
-// stages follow Schied et al. 2017 with FMA contraction (they are checked against the
-// double-precision oracle, not against a float restatement).
+// stages follow Schied et al. 2017 with FMA contraction (checked against the double oracle).
 #include "svgf_internal.hpp"
 
 namespace svgf {
@@ -16,76 +21,106 @@ namespace {
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ char* row_of(const PlaneDev& p, int y) { return p.ptr + (size_t)y * p.pitch; }
+// gfx9 raw buffer descriptor word 3 (32-bit data format, untyped access)
+constexpr int BUF_WORD3 = 0x00020000;
+
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t pitch;
+};
+
+__device__ __forceinline__ Buf bind(const PlaneDev& p, int h)
+{
+    return Buf{__builtin_amdgcn_make_buffer_rsrc(p.ptr, (short)0, (int)(p.pitch * (uint64_t)h), BUF_WORD3),
+               (uint32_t)p.pitch};
+}
+
+__device__ __forceinline__ float4 ld_f4(const Buf& b, uint32_t off)
+{
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(b.r, off, 0, 0));
+}
+__device__ __forceinline__ void st_f4(const Buf& b, uint32_t off, float4 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), b.r, off, 0, 0);
+}
+__device__ __forceinline__ float2 ld_f2(const Buf& b, uint32_t off)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(b.r, off, 0, 0));
+}
+__device__ __forceinline__ void st_f2(const Buf& b, uint32_t off, float2 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), b.r, off, 0, 0);
+}
+__device__ __forceinline__ float ld_f1(const Buf& b, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, off, 0, 0));
+}
+__device__ __forceinline__ void st_f1(const Buf& b, uint32_t off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), b.r, off, 0, 0);
+}
+__device__ __forceinline__ uint32_t ld_u8(const Buf& b, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b8(b.r, off, 0, 0);
+}
+__device__ __forceinline__ void st_u8(const Buf& b, uint32_t off, uint32_t v)
+{
+    __builtin_amdgcn_raw_buffer_store_b8((unsigned char)v, b.r, off, 0, 0);
+}
 
 // Storage traits: FP32 planes (float4 / float2) or FP16 planes (half4 / half2), float math.
 struct F32S {
-    __device__ static __forceinline__ float4 ld4(const PlaneDev& p, int x, int y)
-    {
-        return reinterpret_cast<const float4*>(row_of(p, y))[x];
-    }
-    __device__ static __forceinline__ void st4(const PlaneDev& p, int x, int y, float4 v)
-    {
-        reinterpret_cast<float4*>(row_of(p, y))[x] = v;
-    }
-    __device__ static __forceinline__ float2 ld2(const PlaneDev& p, int x, int y)
-    {
-        return reinterpret_cast<const float2*>(row_of(p, y))[x];
-    }
-    __device__ static __forceinline__ void st2(const PlaneDev& p, int x, int y, float2 v)
-    {
-        reinterpret_cast<float2*>(row_of(p, y))[x] = v;
-    }
-    __device__ static __forceinline__ float lda(const PlaneDev& p, int x, int y)
-    {
-        return reinterpret_cast<const float*>(row_of(p, y))[4 * x + 3];
-    }
+    static constexpr uint32_t B4 = 16, B2 = 8;
+    __device__ static __forceinline__ float4 ld4(const Buf& b, int x, int y) { return ld_f4(b, y * b.pitch + x * B4); }
+    __device__ static __forceinline__ void st4(const Buf& b, int x, int y, float4 v) { st_f4(b, y * b.pitch + x * B4, v); }
+    __device__ static __forceinline__ float2 ld2(const Buf& b, int x, int y) { return ld_f2(b, y * b.pitch + x * B2); }
+    __device__ static __forceinline__ void st2(const Buf& b, int x, int y, float2 v) { st_f2(b, y * b.pitch + x * B2, v); }
+    // alpha channel alone
+    __device__ static __forceinline__ float lda(const Buf& b, int x, int y) { return ld_f1(b, y * b.pitch + x * B4 + 12); }
+    __device__ static __forceinline__ void sta(const Buf& b, int x, int y, float v) { st_f1(b, y * b.pitch + x * B4 + 12, v); }
 };
 
 struct F16S {
-    __device__ static __forceinline__ float4 ld4(const PlaneDev& p, int x, int y)
+    static constexpr uint32_t B4 = 8, B2 = 4;
+    __device__ static __forceinline__ float4 ld4(const Buf& b, int x, int y)
     {
-        half4_t h = reinterpret_cast<const half4_t*>(row_of(p, y))[x];
+        half4_t h = __builtin_bit_cast(half4_t, __builtin_amdgcn_raw_buffer_load_b64(b.r, y * b.pitch + x * B4, 0, 0));
         return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
     }
-    __device__ static __forceinline__ void st4(const PlaneDev& p, int x, int y, float4 v)
+    __device__ static __forceinline__ void st4(const Buf& b, int x, int y, float4 v)
     {
         half4_t h = {(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)v.w};
-        reinterpret_cast<half4_t*>(row_of(p, y))[x] = h;
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, h), b.r, y * b.pitch + x * B4, 0, 0);
     }
-    __device__ static __forceinline__ float2 ld2(const PlaneDev& p, int x, int y)
+    __device__ static __forceinline__ float2 ld2(const Buf& b, int x, int y)
     {
-        half2_t h = reinterpret_cast<const half2_t*>(row_of(p, y))[x];
+        half2_t h = __builtin_bit_cast(half2_t, __builtin_amdgcn_raw_buffer_load_b32(b.r, y * b.pitch + x * B2, 0, 0));
         return make_float2((float)h.x, (float)h.y);
     }
-    __device__ static __forceinline__ void st2(const PlaneDev& p, int x, int y, float2 v)
+    __device__ static __forceinline__ void st2(const Buf& b, int x, int y, float2 v)
     {
         half2_t h = {(_Float16)v.x, (_Float16)v.y};
-        reinterpret_cast<half2_t*>(row_of(p, y))[x] = h;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, h), b.r, y * b.pitch + x * B2, 0, 0);
     }
-    __device__ static __forceinline__ float lda(const PlaneDev& p, int x, int y)
+    __device__ static __forceinline__ float lda(const Buf& b, int x, int y)
     {
-        return (float)reinterpret_cast<const _Float16*>(row_of(p, y))[4 * x + 3];
+        return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(b.r, y * b.pitch + x * B4 + 6, 0, 0));
+    }
+    __device__ static __forceinline__ void sta(const Buf& b, int x, int y, float v)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)v), b.r, y * b.pitch + x * B4 + 6,
+                                              0, 0);
     }
 };
 
-__device__ __forceinline__ float4 ldg4(const PlaneDev& p, int x, int y)
-{
-    return reinterpret_cast<const float4*>(row_of(p, y))[x];
-}
-__device__ __forceinline__ float ldgz(const PlaneDev& p, int x, int y)
-{
-    return reinterpret_cast<const float*>(row_of(p, y))[4 * x + 3];
-}
-__device__ __forceinline__ void stg4(const PlaneDev& p, int x, int y, float4 v)
-{
-    reinterpret_cast<float4*>(row_of(p, y))[x] = v;
-}
-__device__ __forceinline__ float2 ldg2(const PlaneDev& p, int x, int y)
-{
-    return reinterpret_cast<const float2*>(row_of(p, y))[x];
-}
+// float4 G-buffer, float2 motion (FP32 in both storage forms)
+__device__ __forceinline__ float4 ldg4(const Buf& b, int x, int y) { return ld_f4(b, y * b.pitch + x * 16); }
+__device__ __forceinline__ float ldgz(const Buf& b, int x, int y) { return ld_f1(b, y * b.pitch + x * 16 + 12); }
+__device__ __forceinline__ void stg4(const Buf& b, int x, int y, float4 v) { st_f4(b, y * b.pitch + x * 16, v); }
+__device__ __forceinline__ float2 ldg2(const Buf& b, int x, int y) { return ld_f2(b, y * b.pitch + x * 8); }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
@@ -102,9 +137,11 @@ __device__ __forceinline__ float luma_fma(float r, float g, float b)
     return fmaf(b, 0.0722f, fmaf(g, 0.7152f, r * 0.2126f));
 }
 
+constexpr float LOG2E = 1.44269504f;
+
 // exp(x) as the hardware exp2 of a prescaled argument (v_exp_f32); relative error about
 // |x| * 6e-8 from the prescale rounding.
-__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * LOG2E); }
 
 // Exact decision of dot(a, b) >= t for float inputs: each product is split into its rounded
 // value and exact error (FMA), the sum is carried with two-sum error terms, so the sign
@@ -131,6 +168,41 @@ __device__ __forceinline__ bool dot_at_least(float4 a, float4 b, float t)
 }
 
 constexpr float K5[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f, 1.0f / 16.0f};
+// log2(K[i] * K[j]) for the weights folded into one exp2
+__host__ __device__ constexpr float log2k(int i, int j)
+{
+    constexpr float L[5] = {-4.0f, -2.0f, -1.41503751f, -2.0f, -4.0f};  // log2(1/16), log2(4/16), log2(6/16)
+    return L[i] + L[j];
+}
+
+// ----------------------------------------------------------------------------------------
+// Tiles
+// ----------------------------------------------------------------------------------------
+struct Tile {
+    int log2s, rx, ry, i0, j0;
+    __device__ int px(int i) const { return rx + (i << log2s); }
+    __device__ int py(int j) const { return ry + (j << log2s); }
+};
+
+__device__ __forceinline__ Tile decode_tile(int w, uint32_t log2s, uint32_t t)
+{
+    uint32_t s = 1u << log2s;
+    uint32_t lx = ((uint32_t)w + s - 1) >> log2s;
+    uint32_t ntx = (lx + LT - 1) / LT;
+    uint32_t res = t & (s * s - 1);
+    uint32_t tile = t >> (2 * log2s);
+    uint32_t ty = tile / ntx, tx = tile - ty * ntx;
+    return Tile{(int)log2s, (int)(res & (s - 1)), (int)(res >> log2s), (int)(tx * LT), (int)(ty * LT)};
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Renumber so
+// that XCD x runs one contiguous range of logical tiles: neighbouring tiles, which share
+// halo rows and (for s > 1) cache lines of other residues, then meet in the same L2.
+__device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t n)
+{
+    uint32_t q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
 
 // ========================================================================================
 // REFERENCE stages (svgf_denoise.slang)
@@ -138,82 +210,102 @@ constexpr float K5[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f,
 
 // Pass 0 (svgf_denoise.slang:176-191): moments EMA, variance, sqrt into wavelet_a alpha.
 template <class S>
-__device__ __forceinline__ void ref_init_px(const FrameDesc& f, const KParams& p, int x, int y)
+__device__ void ref_init_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
 #pragma clang fp contract(off)
-    float4 L = S::ld4(f.rad, x, y);
+    const int x = T.i0 + (int)(threadIdx.x & (LT - 1)), y = T.j0 + (int)(threadIdx.x >> 4);
+    if (x >= f.w || y >= f.h) return;
+    const Buf rad = bind(f.rad, f.h), mc = bind(f.mom_curr, f.h), wa = bind(f.wa, f.h);
+    float4 L = S::ld4(rad, x, y);
     float l = luma_ref(L.x, L.y, L.z);
     float m1 = 0.0f, m2 = 0.0f;
     if (f.history_valid) {
-        float2 m = S::ld2(f.mom_prev, x, y);
+        float2 m = S::ld2(bind(f.mom_prev, f.h), x, y);
         m1 = m.x;
         m2 = m.y;
     }
     float mu1 = m1 + p.alpha_moments * (l - m1);
     float mu2 = m2 + p.alpha_moments * (l * l - m2);
     float var = fmaxf(mu2 - mu1 * mu1, 0.0f);
-    S::st2(f.mom_curr, x, y, make_float2(mu1, mu2));
-    S::st4(f.wa, x, y, make_float4(L.x, L.y, L.z, __builtin_sqrtf(var + p.variance_eps)));
+    S::st2(mc, x, y, make_float2(mu1, mu2));
+    S::st4(wa, x, y, make_float4(L.x, L.y, L.z, __builtin_sqrtf(var + p.variance_eps)));
 }
 
 // Passes 1..N (svgf_denoise.slang:209-274): 5x5 B3-spline taps at `stride`, clamp-to-edge,
 // weight K[dx] K[dy] exp(-|l_q - l_c| / max(phi * sigma_c, eps)), dy outer / dx inner.
 template <class S>
-__device__ __forceinline__ void ref_atrous_px(const FrameDesc& f, const KParams& p, int x, int y, int pass)
+__device__ void ref_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int pass)
 {
-    const int stride = 1 << (pass - 1);
+    __shared__ float4 sA[LREG * LREG];  // (r, g, b, luminance) of the tile + halo
     const bool source_is_a = (pass & 1) == 1;
-    const PlaneDev& src = source_is_a ? f.wa : f.wb;
-    const bool final_pass = pass == p.iterations;
+    const Buf src = bind(source_is_a ? f.wa : f.wb, f.h);
     const int max_x = f.w - 1, max_y = f.h - 1;
-
-    float4 c = S::ld4(src, x, y);
-    float lc = luma_ref(c.x, c.y, c.z);
-    float sigma = fmaxf(p.phi_luma * c.w, p.variance_eps);
-    float nrs = -1.0f / sigma;
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
+    const int tid = (int)threadIdx.x;
+    for (int e = tid; e < LREG * LREG; e += BLOCK) {
+        int li = e % LREG, lj = e / LREG;
+        int qx = clampi(T.px(T.i0 - LHALO + li), 0, max_x), qy = clampi(T.py(T.j0 - LHALO + lj), 0, max_y);
+        float4 v = S::ld4(src, qx, qy);
+        sA[e] = make_float4(v.x, v.y, v.z, luma_ref(v.x, v.y, v.z));
+    }
+    __syncthreads();
+    const int ti = tid & (LT - 1), tj = tid >> 4;
+    const int x = T.px(T.i0 + ti), y = T.py(T.j0 + tj);
+    if (x <= max_x && y <= max_y) {
+        const int c0 = (tj + LHALO) * LREG + ti + LHALO;
+        float4 c = sA[c0];
+        float cw = S::lda(src, x, y);
+        float lc = c.w;
+        float sigma = fmaxf(p.phi_luma * cw, p.variance_eps);
+        float nrs = -1.0f / sigma;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
 #pragma unroll
-    for (int dy = -2; dy <= 2; ++dy) {
-        int sy = clampi(y + dy * stride, 0, max_y);
+        for (int dy = -2; dy <= 2; ++dy) {
 #pragma unroll
-        for (int dx = -2; dx <= 2; ++dx) {
-            int sx = clampi(x + dx * stride, 0, max_x);
-            float4 v = S::ld4(src, sx, sy);
-            float l = luma_ref(v.x, v.y, v.z);
-            float tw = (K5[dx + 2] * K5[dy + 2]) * fast_exp(fabsf(l - lc) * nrs);
-            s0 = fmaf(v.x, tw, s0);
-            s1 = fmaf(v.y, tw, s1);
-            s2 = fmaf(v.z, tw, s2);
-            ws += tw;
+            for (int dx = -2; dx <= 2; ++dx) {
+                float4 v = sA[c0 + dy * LREG + dx];
+                float tw = (K5[dx + 2] * K5[dy + 2]) * fast_exp(fabsf(v.w - lc) * nrs);
+                s0 = fmaf(v.x, tw, s0);
+                s1 = fmaf(v.y, tw, s1);
+                s2 = fmaf(v.z, tw, s2);
+                ws += tw;
+            }
+        }
+        float4 o;
+        if (ws > 1e-6f) o = make_float4(s0 / ws, s1 / ws, s2 / ws, cw);
+        else o = make_float4(c.x, c.y, c.z, cw);
+        if (pass == p.iterations) {
+            o.w = 1.0f;
+            S::st4(bind(f.den, f.h), x, y, o);
+        } else {
+            S::st4(bind(source_is_a ? f.wb : f.wa, f.h), x, y, o);
         }
     }
-    float4 o;
-    if (ws > 1e-6f) {
-        o = make_float4(s0 / ws, s1 / ws, s2 / ws, c.w);
-    } else {
-        o = make_float4(c.x, c.y, c.z, c.w);
-    }
-    if (final_pass) {
-        o.w = 1.0f;
-        S::st4(f.den, x, y, o);
-    } else {
-        S::st4(source_is_a ? f.wb : f.wa, x, y, o);
-    }
+    __syncthreads();  // the next tile of a grid-stride loop reuses sA
 }
 
 // ========================================================================================
 // CANONICAL stages (Schied et al. 2017)
 // ========================================================================================
 
-// Temporal reprojection + colour / moments accumulation. Writes ctmp = (colour, temporal
-// variance), nlen = history length, mom_curr; the batched form also keeps G for next frame.
+// The wavelet intermediates of the canonical chain (W0..W4: colour, sigma) stay FP32 in both
+// storage forms: the a-trous iterations are bound by arithmetic, not by these bytes, and an
+// FP16 round trip per iteration would add five half-ulp errors to the output.
+using WavS = F32S;
+
+// Temporal reprojection + colour / moments accumulation. Writes W0 = (colour, sqrt(temporal
+// variance)) into wavelet_a, the history length (uint8) and the moments; the batched form
+// also keeps G for the next frame. Standard deviations rather than variances are stored so
+// that heavy-tailed radiance stays inside the FP16 range.
 template <class S, bool COMP>
-__device__ __forceinline__ void can_temporal_px(const FrameDesc& f, const KParams& p, int x, int y)
+__device__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
-    float4 R = S::ld4(f.rad, x, y);
+    const int x = T.i0 + (int)(threadIdx.x & (LT - 1)), y = T.j0 + (int)(threadIdx.x >> 4);
+    if (x >= f.w || y >= f.h) return;
+    const Buf rad = bind(f.rad, f.h), gb = bind(f.g, f.h);
+    float4 R = S::ld4(rad, x, y);
     float l = luma_fma(R.x, R.y, R.z);
-    float4 G = ldg4(f.g, x, y);
-    if (f.gsave.ptr) stg4(f.gsave, x, y, G);
+    float4 G = ldg4(gb, x, y);
+    if (f.gsave.ptr) stg4(bind(f.gsave, f.h), x, y, G);
 
     float sumb = 0.0f, h0 = 0.0f, h1 = 0.0f, h2 = 0.0f, hm1 = 0.0f, hm2 = 0.0f;
     // Compensated form: per-tap (weight, mean, second moment), weight 0 for unused taps so
@@ -221,37 +313,48 @@ __device__ __forceinline__ void can_temporal_px(const FrameDesc& f, const KParam
     float tb[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tmu[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tm2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     int nmax = 0;
     if (f.history_valid) {
-        float2 m = ldg2(f.mv, x, y);
+        const Buf mvb = bind(f.mv, f.h), gp = bind(f.gprev, f.h), hp = bind(f.hist_prev, f.h), mp = bind(f.mom_prev, f.h);
+        float2 m = ldg2(mvb, x, y);
         float flx = floorf(m.x), fly = floorf(m.y);
         float fx = m.x - flx, fy = m.y - fly;
         int x0 = x + (int)flx, y0 = y + (int)fly;
+        // Issue the G / history / moment loads of all four taps at once (addresses clamped into
+        // the frame; taps outside it are discarded below) so the reprojection costs one
+        // dependent memory round trip instead of three.
+        float4 Gq[4], H[4];
+        float2 M[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            int cx = clampi(x0 + (t & 1), 0, f.w - 1), cy = clampi(y0 + (t >> 1), 0, f.h - 1);
+            Gq[t] = ldg4(gp, cx, cy);
+            H[t] = S::ld4(hp, cx, cy);
+            M[t] = S::ld2(mp, cx, cy);
+        }
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
 #pragma unroll
             for (int ii = 0; ii < 2; ++ii) {
+                const int t = jj * 2 + ii;
                 float b = (ii ? fx : 1.0f - fx) * (jj ? fy : 1.0f - fy);
                 int qx = x0 + ii, qy = y0 + jj;
                 if (!(b > 0.0f) || qx < 0 || qx >= f.w || qy < 0 || qy >= f.h) continue;
-                float4 Gq = ldg4(f.gprev, qx, qy);
-                float dz = fabsf(Gq.w - G.w);
+                float dz = fabsf(Gq[t].w - G.w);
                 if (__builtin_fmaf(-p.depth_reject, G.w, dz) > 0.0f) continue;
-                if (!dot_at_least(Gq, G, p.normal_reject)) continue;
-                float4 H = S::ld4(f.hist_prev, qx, qy);
-                int nq = (int)H.w;
+                if (!dot_at_least(Gq[t], G, p.normal_reject)) continue;
+                int nq = (int)H[t].w;
                 if (nq < 1) continue;
                 sumb += b;
-                h0 = fmaf(b, H.x, h0);
-                h1 = fmaf(b, H.y, h1);
-                h2 = fmaf(b, H.z, h2);
+                h0 = fmaf(b, H[t].x, h0);
+                h1 = fmaf(b, H[t].y, h1);
+                h2 = fmaf(b, H[t].z, h2);
                 nmax = max(nmax, nq);
-                float2 M = S::ld2(f.mom_prev, qx, qy);
                 if (COMP) {
-                    tb[jj * 2 + ii] = b;
-                    tmu[jj * 2 + ii] = M.x;
-                    tm2[jj * 2 + ii] = M.y * M.y;
+                    tb[t] = b;
+                    tmu[t] = M[t].x;
+                    tm2[t] = M[t].y * M[t].y;
                 } else {
-                    hm1 = fmaf(b, M.x, hm1);
-                    hm2 = fmaf(b, M.y, hm2);
+                    hm1 = fmaf(b, M[t].x, hm1);
+                    hm2 = fmaf(b, M[t].y, hm2);
                 }
             }
         }
@@ -289,39 +392,33 @@ __device__ __forceinline__ void can_temporal_px(const FrameDesc& f, const KParam
     float ac = fmaxf(p.alpha_color, rn), am = fmaxf(p.alpha_moments, rn);
     float c0 = fmaf(ac, R.x - h0, h0), c1 = fmaf(ac, R.y - h1, h1), c2 = fmaf(ac, R.z - h2, h2);
     float var;
+    const Buf mc = bind(f.mom_curr, f.h);
     if (COMP) {
         float d = l - hmu;
         float mu = fmaf(am, d, hmu);
         float m2 = (1.0f - am) * fmaf(am * d, d, hvar);
-        S::st2(f.mom_curr, x, y, make_float2(mu, __builtin_sqrtf(m2)));
+        S::st2(mc, x, y, make_float2(mu, __builtin_sqrtf(m2)));
         var = m2;
     } else {
         float mu1 = fmaf(am, l - hm1, hm1);
         float mu2 = fmaf(am, fmaf(l, l, -hm2), hm2);
-        S::st2(f.mom_curr, x, y, make_float2(mu1, mu2));
+        S::st2(mc, x, y, make_float2(mu1, mu2));
         var = fmaxf(mu2 - mu1 * mu1, 0.0f);
     }
-    S::st4(f.ctmp, x, y, make_float4(c0, c1, c2, var));
-    reinterpret_cast<uint8_t*>(row_of(f.nlen, y))[x] = (uint8_t)n_new;
+    WavS::st4(bind(f.wa, f.h), x, y, make_float4(c0, c1, c2, __builtin_sqrtf(var)));
+    const Buf nl = bind(f.nlen, f.h);
+    st_u8(nl, y * nl.pitch + x, (uint32_t)n_new);
 }
 
-// Depth-gradient (central differences of clamped neighbours, halved).
-__device__ __forceinline__ void depth_gradient(const FrameDesc& f, int x, int y, float& gx, float& gy)
-{
-    float zl = ldgz(f.g, clampi(x - 1, 0, f.w - 1), y), zr = ldgz(f.g, clampi(x + 1, 0, f.w - 1), y);
-    float zu = ldgz(f.g, x, clampi(y - 1, 0, f.h - 1)), zd = ldgz(f.g, x, clampi(y + 1, 0, f.h - 1));
-    gx = 0.5f * (zr - zl);
-    gy = 0.5f * (zd - zu);
-}
-
-// log2 of the depth / normal edge-stop factor: -|dz| / (phi_z |g.o| + eps) * log2(e)
-// + phi_n * log2(max(0, n.n')). Returns -inf when the normal term is zero.
+// log2 of the depth / normal edge-stop factor between the centre (Gc, depth gradient g) and a
+// tap at pixel offset (ox, oy): -|dz| / (phi_z |g.o| + eps) * log2(e) + phi_n * log2(n.n').
+// Returns -inf when the normal term is zero.
 __device__ __forceinline__ float edge_log2(const KParams& p, float4 Gc, float4 Gq, float gx, float gy, float ox, float oy)
 {
     float a = 0.0f;
     if (p.phi_depth > 0.0f) {
         float den = fmaf(p.phi_depth, fabsf(fmaf(gx, ox, gy * oy)), p.depth_eps);
-        a = -fabsf(Gq.w - Gc.w) * __builtin_amdgcn_rcpf(den) * 1.44269504f;
+        a = -fabsf(Gq.w - Gc.w) * __builtin_amdgcn_rcpf(den) * LOG2E;
     }
     if (p.phi_normal > 0.0f) {
         float d = fmaf(Gc.x, Gq.x, fmaf(Gc.y, Gq.y, Gc.z * Gq.z));
@@ -330,116 +427,159 @@ __device__ __forceinline__ float edge_log2(const KParams& p, float4 Gc, float4 G
     return a;
 }
 
-// Spatial variance where the history is young (n < 4), else the temporal variance.
-template <class S, bool COMP>
-__device__ __forceinline__ void can_variance_px(const FrameDesc& f, const KParams& p, int x, int y)
-{
-    float4 C = S::ld4(f.ctmp, x, y);
-    int n = reinterpret_cast<const uint8_t*>(row_of(f.nlen, y))[x];
-    float var = C.w;
-    if (n < 4) {
-        float4 G = ldg4(f.g, x, y);
-        float gx, gy;
-        depth_gradient(f, x, y, gx, gy);
-        float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
-#pragma unroll
-        for (int dy = -1; dy <= 1; ++dy) {
-            int qy = clampi(y + dy, 0, f.h - 1);
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                int qx = clampi(x + dx, 0, f.w - 1);
-                int t = (dy + 1) * 3 + dx + 1;
-                float4 Gq = ldg4(f.g, qx, qy);
-                float e = edge_log2(p, G, Gq, gx, gy, (float)dx, (float)dy);
-                float w = __builtin_amdgcn_exp2f(e);
-                float4 Cq = S::ld4(f.ctmp, qx, qy);
-                tw[t] = w;
-                tl[t] = luma_fma(Cq.x, Cq.y, Cq.z);
-                sw += w;
-                s1 = fmaf(w, tl[t], s1);
-            }
-        }
-        if (sw > 0.0f) {
-            float inv = 1.0f / sw;
-            float mean = s1 * inv, s2 = 0.0f;
-#pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                float d = tl[t] - mean;
-                s2 = fmaf(tw[t] * d, d, s2);
-            }
-            var = s2 * inv;
-        } else {
-            var = 0.0f;
-        }
-    }
-    S::st4(f.wa, x, y, make_float4(C.x, C.y, C.z, var));
-}
-
-// A-trous iteration k (stride 2^k) with luminance, depth and normal edge-stops; variance
-// filtered by squared weights. Iteration 0 also writes the colour history.
+// Spatial variance where the history is young (n < 4): patches the variance channel of W0 in
+// place with the edge-aware 3x3 variance of the temporal luminance. Older pixels keep the
+// temporal variance the temporal stage wrote. (Only the alpha bytes are written; neighbours
+// read only colour bytes, so the in-place update has no read/write overlap.)
 template <class S>
-__device__ __forceinline__ void can_atrous_px(const FrameDesc& f, const KParams& p, int x, int y, int k)
+__device__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
-    const int s = 1 << k;
-    const PlaneDev& src = (k & 1) ? f.wb : f.wa;
-    const bool fin = k == p.iterations - 1;
-    const int max_x = f.w - 1, max_y = f.h - 1;
-
-    float4 c = S::ld4(src, x, y);
-    float lc = luma_fma(c.x, c.y, c.z);
-    float gv = 0.0f;
+    const int x = T.i0 + (int)(threadIdx.x & (LT - 1)), y = T.j0 + (int)(threadIdx.x >> 4);
+    if (x >= f.w || y >= f.h) return;
+    const Buf nl = bind(f.nlen, f.h);
+    int n = (int)ld_u8(nl, y * nl.pitch + x);
+    if (n >= 4) return;
+    const Buf gb = bind(f.g, f.h), wa = bind(f.wa, f.h);
+    float4 G = ldg4(gb, x, y);
+    float zl = ldgz(gb, clampi(x - 1, 0, f.w - 1), y), zr = ldgz(gb, clampi(x + 1, 0, f.w - 1), y);
+    float zu = ldgz(gb, x, clampi(y - 1, 0, f.h - 1)), zd = ldgz(gb, x, clampi(y + 1, 0, f.h - 1));
+    float gx = 0.5f * (zr - zl), gy = 0.5f * (zd - zu);
+    float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy) {
-        int qy = clampi(y + dy, 0, max_y);
-        float ky = dy == 0 ? 0.5f : 0.25f;
+        int qy = clampi(y + dy, 0, f.h - 1);
 #pragma unroll
         for (int dx = -1; dx <= 1; ++dx) {
-            int qx = clampi(x + dx, 0, max_x);
-            float kx = dx == 0 ? 0.5f : 0.25f;
-            gv = fmaf(kx * ky, S::lda(src, qx, qy), gv);
-        }
-    }
-    float sigma = p.phi_luma * __builtin_sqrtf(fmaxf(gv, 0.0f) + p.variance_eps);
-    float nls = -1.44269504f / sigma;
-    float4 G = ldg4(f.g, x, y);
-    float gx, gy;
-    depth_gradient(f, x, y, gx, gy);
-
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sw = 0.0f, sv = 0.0f;
-#pragma unroll
-    for (int dy = -2; dy <= 2; ++dy) {
-        int qy = clampi(y + dy * s, 0, max_y);
-#pragma unroll
-        for (int dx = -2; dx <= 2; ++dx) {
-            int qx = clampi(x + dx * s, 0, max_x);
-            float4 v = S::ld4(src, qx, qy);
-            float4 Gq = ldg4(f.g, qx, qy);
-            float e = edge_log2(p, G, Gq, gx, gy, (float)(dx * s), (float)(dy * s));
-            e = fmaf(fabsf(luma_fma(v.x, v.y, v.z) - lc), nls, e);
-            float w = (K5[dx + 2] * K5[dy + 2]) * __builtin_amdgcn_exp2f(e);
-            s0 = fmaf(w, v.x, s0);
-            s1 = fmaf(w, v.y, s1);
-            s2 = fmaf(w, v.z, s2);
+            int qx = clampi(x + dx, 0, f.w - 1);
+            int t = (dy + 1) * 3 + dx + 1;
+            float4 Gq = ldg4(gb, qx, qy);
+            float w = __builtin_amdgcn_exp2f(edge_log2(p, G, Gq, gx, gy, (float)dx, (float)dy));
+            float4 Cq = WavS::ld4(wa, qx, qy);
+            tw[t] = w;
+            tl[t] = luma_fma(Cq.x, Cq.y, Cq.z);
             sw += w;
-            sv = fmaf(w * w, v.w, sv);
+            s1 = fmaf(w, tl[t], s1);
         }
     }
-    float4 o;
-    if (sw > 1e-6f) {
+    float var = 0.0f;
+    if (sw > 0.0f) {
         float inv = 1.0f / sw;
-        o = make_float4(s0 * inv, s1 * inv, s2 * inv, sv * inv * inv);
-    } else {
-        o = c;
+        float mean = s1 * inv, s2 = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            float d = tl[t] - mean;
+            s2 = fmaf(tw[t] * d, d, s2);
+        }
+        var = s2 * inv;
     }
-    if (k == 0) {
-        float n = (float)reinterpret_cast<const uint8_t*>(row_of(f.nlen, y))[x];
-        S::st4(f.hist_curr, x, y, make_float4(o.x, o.y, o.z, n));
+    WavS::sta(wa, x, y, __builtin_sqrtf(var));
+}
+
+// A-trous iteration k (stride s = 2^k) on one lattice tile: luminance, depth and normal
+// edge-stops, variance filtered by squared weights. Iteration 0 also writes the colour
+// history, the last one the output.
+template <class S>
+__device__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
+{
+    __shared__ float4 sA[LREG * LREG];  // (r, g, b, luminance)
+    __shared__ float4 sB[LREG * LREG];  // (normal, depth)
+    __shared__ float sV[LREG * LREG];   // variance
+    const int s = 1 << k;
+    const Buf src = bind((k & 1) ? f.wb : f.wa, f.h), gb = bind(f.g, f.h);
+    const int max_x = f.w - 1, max_y = f.h - 1;
+    const int tid = (int)threadIdx.x;
+    for (int e = tid; e < LREG * LREG; e += BLOCK) {
+        int li = e % LREG, lj = e / LREG;
+        int qx = clampi(T.px(T.i0 - LHALO + li), 0, max_x), qy = clampi(T.py(T.j0 - LHALO + lj), 0, max_y);
+        float4 v = WavS::ld4(src, qx, qy);
+        sA[e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
+        sB[e] = ldg4(gb, qx, qy);
+        sV[e] = v.w * v.w;
     }
-    if (fin) {
-        S::st4(f.den, x, y, make_float4(o.x, o.y, o.z, 1.0f));
-    } else {
-        S::st4((k & 1) ? f.wa : f.wb, x, y, o);
+    __syncthreads();
+    const int ti = tid & (LT - 1), tj = tid >> 4;
+    const int x = T.px(T.i0 + ti), y = T.py(T.j0 + tj);
+    if (x <= max_x && y <= max_y) {
+        const int c0 = (tj + LHALO) * LREG + ti + LHALO;
+        const float4 C = sA[c0], Gc = sB[c0];
+        // 3x3 binomial of the variance and the depth gradient at pixel distance 1
+        float gv, gx, gy;
+        if (s == 1) {
+            gv = 0.0f;
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx)
+                    gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c0 + dy * LREG + dx], gv);
+            gx = 0.5f * (sB[c0 + 1].w - sB[c0 - 1].w);
+            gy = 0.5f * (sB[c0 + LREG].w - sB[c0 - LREG].w);
+        } else {
+            const int xl = max(x - 1, 0), xr = min(x + 1, max_x), yu = max(y - 1, 0), yd = min(y + 1, max_y);
+            const int xs[3] = {xl, x, xr}, ys[3] = {yu, y, yd};
+            gv = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    float v;
+                    if (a == 1 && b == 1) {
+                        v = sV[c0];
+                    } else {
+                        float sd = WavS::lda(src, xs[b], ys[a]);
+                        v = sd * sd;
+                    }
+                    gv = fmaf((b == 1 ? 0.5f : 0.25f) * (a == 1 ? 0.5f : 0.25f), v, gv);
+                }
+            gx = 0.5f * (ldgz(gb, xr, y) - ldgz(gb, xl, y));
+            gy = 0.5f * (ldgz(gb, x, yd) - ldgz(gb, x, yu));
+        }
+        const float nls = -LOG2E / (p.phi_luma * __builtin_sqrtf(fmaxf(gv, 0.0f) + p.variance_eps));
+        // -log2(e) / (phi_z |g.o| + eps) for the 12 offsets after the centre in row-major
+        // order; the mirrored offset has the bitwise-same denominator.
+        float nrd[12];
+#pragma unroll
+        for (int i = 0; i < 12; ++i) {
+            const int dx = (i + 13) % 5 - 2, dy = (i + 13) / 5 - 2;
+            float den = fmaf(p.phi_depth, fabsf(fmaf(gx, (float)(dx * s), gy * (float)(dy * s))), p.depth_eps);
+            nrd[i] = p.phi_depth > 0.0f ? -LOG2E * __builtin_amdgcn_rcpf(den) : 0.0f;
+        }
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sw = 0.0f, sv = 0.0f;
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+            for (int dx = -2; dx <= 2; ++dx) {
+                const int o = (dy + 2) * 5 + dx + 2;  // 0..24, centre 12
+                const int q = c0 + dy * LREG + dx;
+                const float4 A = sA[q], B = sB[q];
+                float e = log2k(dx + 2, dy + 2);
+                if (o != 12) e = fmaf(fabsf(B.w - Gc.w), nrd[o > 12 ? o - 13 : 11 - o], e);
+                e = fmaf(fabsf(A.w - C.w), nls, e);
+                float d = fmaf(Gc.x, B.x, fmaf(Gc.y, B.y, Gc.z * B.z));
+                e = fmaf(p.phi_normal, __builtin_amdgcn_logf(fmaxf(d, 1e-37f)), e);
+                float w = __builtin_amdgcn_exp2f(e);
+                s0 = fmaf(w, A.x, s0);
+                s1 = fmaf(w, A.y, s1);
+                s2 = fmaf(w, A.z, s2);
+                sw += w;
+                sv = fmaf(w * w, sV[q], sv);
+            }
+        }
+        float4 o;
+        if (sw > 1e-6f) {
+            float inv = 1.0f / sw;
+            o = make_float4(s0 * inv, s1 * inv, s2 * inv, __builtin_sqrtf(sv * inv * inv));
+        } else {
+            o = make_float4(C.x, C.y, C.z, __builtin_sqrtf(sV[c0]));
+        }
+        if (k == 0) {
+            const Buf nl = bind(f.nlen, f.h);
+            float n = (float)ld_u8(nl, y * nl.pitch + x);
+            S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, n));
+        }
+        if (k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
+        else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
     }
+    __syncthreads();  // the next tile of a grid-stride loop reuses the LDS arrays
 }
 
 // ========================================================================================
@@ -448,58 +588,63 @@ __device__ __forceinline__ void can_atrous_px(const FrameDesc& f, const KParams&
 enum StageId : int { ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_VARIANCE = 3, ST_CAN_ATROUS = 4 };
 
 template <class S, int STAGE, bool COMP>
-__device__ __forceinline__ void run_stage_px(const FrameDesc& f, const KParams& p, int x, int y, int arg)
+__device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, const Tile& T, int arg)
 {
-    if constexpr (STAGE == ST_REF_INIT) ref_init_px<S>(f, p, x, y);
-    else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_px<S>(f, p, x, y, arg);
-    else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_px<S, COMP>(f, p, x, y);
-    else if constexpr (STAGE == ST_CAN_VARIANCE) can_variance_px<S, COMP>(f, p, x, y);
-    else can_atrous_px<S>(f, p, x, y, arg);
+    if constexpr (STAGE == ST_REF_INIT) ref_init_tile<S>(f, p, T);
+    else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_tile<S>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, COMP>(f, p, T);
+    else if constexpr (STAGE == ST_CAN_VARIANCE) can_variance_tile<S>(f, p, T);
+    else can_atrous_tile<S>(f, p, T, arg);
 }
 
+// arg: REFERENCE a-trous pass (1..N) or CANONICAL a-trous iteration (0..N-1); log2s: stride.
 template <class S, int STAGE, bool COMP>
-__global__ __launch_bounds__(TILE_W* TILE_H) void k_frame(FrameDesc f, KParams p, int arg)
+__global__ __launch_bounds__(BLOCK) void k_frame(FrameDesc f, KParams p, int arg, uint32_t log2s)
 {
-    int x = blockIdx.x * TILE_W + threadIdx.x;
-    int y = blockIdx.y * TILE_H + threadIdx.y;
-    if (x >= f.w || y >= f.h) return;
-    run_stage_px<S, STAGE, COMP>(f, p, x, y, arg);
+    Tile T = decode_tile(f.w, log2s, xcd_logical(blockIdx.x, gridDim.x));
+    run_stage<S, STAGE, COMP>(f, p, T, arg);
 }
 
+// One block per tile of the batch (the host sizes the grid to the stage's tile count; a
+// grid-stride loop here would keep a second tile's state live and double the VGPR count).
 template <class S, int STAGE, bool COMP>
-__global__ __launch_bounds__(TILE_W* TILE_H) void k_batch(BatchTable bt, KParams p, int arg)
+__global__ __launch_bounds__(BLOCK) void k_batch(BatchTable bt, KParams p, int arg, uint32_t log2s, int stage)
 {
-    const uint32_t nf = bt.hdr[0], nt = bt.hdr[1];
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        // largest frame index i with starts[i] <= t
-        uint32_t lo = 0, hi = nf - 1;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (bt.starts[mid] <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const FrameDesc& f = bt.frames[lo];
-        uint32_t local = t - bt.starts[lo];
-        uint32_t ty = local / (uint32_t)f.tiles_x, tx = local - ty * (uint32_t)f.tiles_x;
-        int x = (int)tx * TILE_W + threadIdx.x;
-        int y = (int)ty * TILE_H + threadIdx.y;
-        if (x < f.w && y < f.h) run_stage_px<S, STAGE, COMP>(f, p, x, y, arg);
+    const uint32_t nf = bt.hdr[0];
+    const uint32_t* starts = bt.starts + (uint32_t)stage * STARTS_ROW;
+    const uint32_t t = xcd_logical(blockIdx.x, gridDim.x);
+    if (t >= starts[nf]) return;
+    // largest frame index i with starts[i] <= t
+    uint32_t lo = 0, hi = nf - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= t) lo = mid;
+        else hi = mid - 1;
     }
+    // frame descriptor into scalar registers (the index is uniform across the block)
+    FrameDesc f;
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(bt.frames + lo);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&f);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(FrameDesc) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    }
+    Tile T = decode_tile(f.w, log2s, t - starts[lo]);
+    run_stage<S, STAGE, COMP>(f, p, T, arg);
 }
 
 template <class S, int STAGE, bool COMP>
-void launch_frame(const FrameDesc& f, const KParams& p, int arg, hipStream_t stream)
+void launch_frame(const FrameDesc& f, const KParams& p, int arg, uint32_t log2s, hipStream_t stream)
 {
-    dim3 grid((f.w + TILE_W - 1) / TILE_W, (f.h + TILE_H - 1) / TILE_H);
-    dim3 block(TILE_W, TILE_H);
-    hipLaunchKernelGGL((k_frame<S, STAGE, COMP>), grid, block, 0, stream, f, p, arg);
+    uint32_t n = tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s);
+    hipLaunchKernelGGL((k_frame<S, STAGE, COMP>), dim3(n), dim3(BLOCK), 0, stream, f, p, arg, log2s);
 }
 
 template <class S, int STAGE, bool COMP>
-void launch_batch(const BatchTable& bt, const KParams& p, int arg, int grid_blocks, hipStream_t stream)
+void launch_batch(const BatchTable& bt, const KParams& p, int arg, uint32_t log2s, int stage, const uint32_t* grids,
+                  hipStream_t stream)
 {
-    dim3 block(TILE_W, TILE_H);
-    hipLaunchKernelGGL((k_batch<S, STAGE, COMP>), dim3(grid_blocks), block, 0, stream, bt, p, arg);
+    hipLaunchKernelGGL((k_batch<S, STAGE, COMP>), dim3(grids[stage]), dim3(BLOCK), 0, stream, bt, p, arg, log2s, stage);
 }
 
 inline void mark(hipEvent_t* ev, int i, hipStream_t s)
@@ -511,10 +656,10 @@ template <class S>
 void reference_frame(const FrameDesc& f, const KParams& p, hipStream_t s, hipEvent_t* ev)
 {
     mark(ev, 0, s);
-    launch_frame<S, ST_REF_INIT, false>(f, p, 0, s);
+    launch_frame<S, ST_REF_INIT, false>(f, p, 0, 0, s);
     mark(ev, 1, s);
     for (int pass = 1; pass <= p.iterations; ++pass) {
-        launch_frame<S, ST_REF_ATROUS, false>(f, p, pass, s);
+        launch_frame<S, ST_REF_ATROUS, false>(f, p, pass, (uint32_t)(pass - 1), s);
         mark(ev, pass + 1, s);
     }
 }
@@ -523,38 +668,38 @@ template <class S, bool COMP>
 void canonical_frame(const FrameDesc& f, const KParams& p, hipStream_t s, hipEvent_t* ev)
 {
     mark(ev, 0, s);
-    launch_frame<S, ST_CAN_TEMPORAL, COMP>(f, p, 0, s);
+    launch_frame<S, ST_CAN_TEMPORAL, COMP>(f, p, 0, 0, s);
     mark(ev, 1, s);
-    launch_frame<S, ST_CAN_VARIANCE, COMP>(f, p, 0, s);
+    launch_frame<S, ST_CAN_VARIANCE, COMP>(f, p, 0, 0, s);
     mark(ev, 2, s);
     for (int k = 0; k < p.iterations; ++k) {
-        launch_frame<S, ST_CAN_ATROUS, COMP>(f, p, k, s);
+        launch_frame<S, ST_CAN_ATROUS, COMP>(f, p, k, (uint32_t)k, s);
         mark(ev, k + 3, s);
     }
 }
 
 template <class S>
-void reference_batch(const BatchTable& bt, const KParams& p, int g, hipStream_t s, hipEvent_t* ev)
+void reference_batch(const BatchTable& bt, const KParams& p, const uint32_t* g, hipStream_t s, hipEvent_t* ev)
 {
     mark(ev, 0, s);
-    launch_batch<S, ST_REF_INIT, false>(bt, p, 0, g, s);
+    launch_batch<S, ST_REF_INIT, false>(bt, p, 0, 0, 0, g, s);
     mark(ev, 1, s);
     for (int pass = 1; pass <= p.iterations; ++pass) {
-        launch_batch<S, ST_REF_ATROUS, false>(bt, p, pass, g, s);
+        launch_batch<S, ST_REF_ATROUS, false>(bt, p, pass, (uint32_t)(pass - 1), pass, g, s);
         mark(ev, pass + 1, s);
     }
 }
 
 template <class S, bool COMP>
-void canonical_batch(const BatchTable& bt, const KParams& p, int g, hipStream_t s, hipEvent_t* ev)
+void canonical_batch(const BatchTable& bt, const KParams& p, const uint32_t* g, hipStream_t s, hipEvent_t* ev)
 {
     mark(ev, 0, s);
-    launch_batch<S, ST_CAN_TEMPORAL, COMP>(bt, p, 0, g, s);
+    launch_batch<S, ST_CAN_TEMPORAL, COMP>(bt, p, 0, 0, 0, g, s);
     mark(ev, 1, s);
-    launch_batch<S, ST_CAN_VARIANCE, COMP>(bt, p, 0, g, s);
+    launch_batch<S, ST_CAN_VARIANCE, COMP>(bt, p, 0, 0, 1, g, s);
     mark(ev, 2, s);
     for (int k = 0; k < p.iterations; ++k) {
-        launch_batch<S, ST_CAN_ATROUS, COMP>(bt, p, k, g, s);
+        launch_batch<S, ST_CAN_ATROUS, COMP>(bt, p, k, (uint32_t)k, k + 2, g, s);
         mark(ev, k + 3, s);
     }
 }
@@ -580,7 +725,7 @@ hipError_t launch_canonical_frame(const FrameDesc& f, const KParams& p, Storage 
     return hipGetLastError();
 }
 
-hipError_t launch_reference_batch(const BatchTable& bt, const KParams& p, Storage st, int grid_blocks,
+hipError_t launch_reference_batch(const BatchTable& bt, const KParams& p, Storage st, const uint32_t* grid_blocks,
                                   hipStream_t stream, hipEvent_t* ev)
 {
     if (st == Storage::F32) reference_batch<F32S>(bt, p, grid_blocks, stream, ev);
@@ -588,7 +733,7 @@ hipError_t launch_reference_batch(const BatchTable& bt, const KParams& p, Storag
     return hipGetLastError();
 }
 
-hipError_t launch_canonical_batch(const BatchTable& bt, const KParams& p, Storage st, int grid_blocks,
+hipError_t launch_canonical_batch(const BatchTable& bt, const KParams& p, Storage st, const uint32_t* grid_blocks,
                                   hipStream_t stream, hipEvent_t* ev)
 {
     if (st == Storage::F32) {

@@ -111,6 +111,37 @@ class SynthDesc(ctypes.Structure):
     ]
 
 
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("lanes", ctypes.c_uint32), ("max_batch_frames", ctypes.c_uint32),
+                ("max_batch_pixels", ctypes.c_uint64), ("max_wait_us", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("stream_id", ctypes.c_uint32), ("status", ctypes.c_int32),
+                ("submit_ns", ctypes.c_uint64), ("launch_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64),
+                ("batch_frames", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class SimDesc(ctypes.Structure):
+    _fields_ = [("streams", ctypes.c_uint32), ("n_classes", ctypes.c_uint32),
+                ("class_width", ctypes.POINTER(ctypes.c_uint32)), ("class_height", ctypes.POINTER(ctypes.c_uint32)),
+                ("class_inputs", ctypes.POINTER(Request)), ("inputs_per_class", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("rate_hz", ctypes.c_double), ("duration_s", ctypes.c_double),
+                ("churn_per_s", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class SimResult(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "rejected", "batches", "streams_opened",
+                                                "streams_closed", "pixels")] + \
+               [(n, ctypes.c_double) for n in ("wall_s", "mpx_per_s", "latency_p50_ms", "latency_p99_ms",
+                                               "latency_max_ms", "latency_mean_ms", "mean_batch_frames",
+                                               "offered_mpx_per_s")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -154,6 +185,14 @@ def lib():
             "svgf_denoise_batch": [vp, P(Request), ctypes.c_uint32, vp],
             "svgf_synth_frame": [P(SynthDesc), vp, vp, vp],
             "svgf_stage_times": [vp, P(ctypes.c_float), ctypes.c_uint32, P(ctypes.c_uint32)],
+            "svgf_server_create": [vp, P(ServerDesc), P(vp)],
+            "svgf_server_destroy": [vp],
+            "svgf_server_stream_open": [vp, ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_uint32)],
+            "svgf_server_stream_close": [vp, ctypes.c_uint32],
+            "svgf_server_submit": [vp, P(Request), ctypes.c_uint64],
+            "svgf_server_poll": [vp, P(Completion), ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_uint32],
+            "svgf_server_drain": [vp],
+            "svgf_server_simulate": [vp, P(SimDesc), P(SimResult)],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args
@@ -320,6 +359,75 @@ class Context:
             r.gbuffer = plane_of(it.get("gbuffer"), FMT_RGBA32F)
             r.motion = plane_of(it.get("motion"), FMT_RG32F)
         check(lib().svgf_denoise_batch(self.handle, arr, len(items), _stream_handle(stream)), "svgf_denoise_batch")
+
+
+class Server:
+    """Continuous-batching server over a Context's resident streams (svgf_server_*)."""
+
+    def __init__(self, ctx: Context, lanes=2, max_batch_frames=None, max_batch_pixels=None, max_wait_us=0):
+        d = ServerDesc(lanes, max_batch_frames or ctx.desc.max_batch_frames,
+                       max_batch_pixels or ctx.desc.max_batch_pixels, max_wait_us, 0)
+        h = ctypes.c_void_p()
+        check(lib().svgf_server_create(ctx.handle, ctypes.byref(d), ctypes.byref(h)), "svgf_server_create")
+        self.ctx, self.handle = ctx, h
+
+    def close(self):
+        if self.handle:
+            check(lib().svgf_server_destroy(self.handle), "svgf_server_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def stream_open(self, w, h) -> int:
+        sid = ctypes.c_uint32()
+        check(lib().svgf_server_stream_open(self.handle, w, h, ctypes.byref(sid)), "svgf_server_stream_open")
+        return sid.value
+
+    def stream_close(self, sid):
+        check(lib().svgf_server_stream_close(self.handle, sid), "svgf_server_stream_close")
+
+    def submit(self, item: dict, tag: int):
+        st = self.ctx.storage
+        r = Request()
+        r.stream_id = item["stream_id"]
+        r.radiance = plane_of(item["radiance"], fmt4(st))
+        r.denoised = plane_of(item["denoised"], fmt4(st))
+        r.gbuffer = plane_of(item.get("gbuffer"), FMT_RGBA32F)
+        r.motion = plane_of(item.get("motion"), FMT_RG32F)
+        check(lib().svgf_server_submit(self.handle, ctypes.byref(r), tag), "svgf_server_submit")
+
+    def poll(self, wait_us=0, capacity=256):
+        buf = (Completion * capacity)()
+        n = ctypes.c_uint32()
+        check(lib().svgf_server_poll(self.handle, buf, capacity, ctypes.byref(n), wait_us), "svgf_server_poll")
+        return [buf[i] for i in range(n.value)]
+
+    def drain(self):
+        check(lib().svgf_server_drain(self.handle), "svgf_server_drain")
+
+    def simulate(self, *, streams, classes, class_inputs, rate_hz, duration_s, churn_per_s, seed=0):
+        """classes: list of (w, h); class_inputs: list (per class) of lists of input dicts
+        (radiance, gbuffer, motion tensors)."""
+        st = self.ctx.storage
+        ipc = len(class_inputs[0])
+        arr = (Request * (len(classes) * ipc))()
+        for c, ins in enumerate(class_inputs):
+            assert len(ins) == ipc
+            for k, it in enumerate(ins):
+                r = arr[c * ipc + k]
+                r.radiance = plane_of(it["radiance"], fmt4(st))
+                r.gbuffer = plane_of(it.get("gbuffer"), FMT_RGBA32F)
+                r.motion = plane_of(it.get("motion"), FMT_RG32F)
+        cw = (ctypes.c_uint32 * len(classes))(*[w for w, _ in classes])
+        ch = (ctypes.c_uint32 * len(classes))(*[h for _, h in classes])
+        d = SimDesc(streams, len(classes), cw, ch, arr, ipc, 0, rate_hz, duration_s, churn_per_s, seed)
+        res = SimResult()
+        check(lib().svgf_server_simulate(self.handle, ctypes.byref(d), ctypes.byref(res)), "svgf_server_simulate")
+        return res.as_dict()
 
 
 def _stream_handle(stream):
