@@ -320,7 +320,8 @@ int oracle_r_frame_f64(int w, int h, const oracle_params* p, int history_valid,
  *   depth and normal edge-stops (stride 1): var = sum w (l_q - mean)^2 / sum w; else the
  *   temporal variance. Output W0 = (colour, var).
  * A-trous iteration k (stride s = 2^k), taps as svgf_denoise.slang:225-254:
- *   gv = 3x3 binomial (1,2,1)/4 x (1,2,1)/4 of the variance around p (clamped)
+ *   gv = 3x3 binomial (1,2,1)/4 x (1,2,1)/4 of the variance at p + (i s, j s), i, j in
+ *        {-1, 0, 1} (clamped): the prefilter runs on the iteration's own lattice
  *   sigma_l = phi_luma * sqrt(max(gv, 0) + variance_eps)
  *   (gx, gy) = central differences of depth at p (clamped neighbours, halved)
  *   w = K[dx] K[dy] exp(-|l_q - l_p| / sigma_l - |z_q - z_p| / (phi_depth |gx dx s + gy dy s| + depth_eps))
@@ -560,7 +561,7 @@ static void c_atrous_rows(const c_job* J)
                 static const double K3[3] = {0.25, 0.5, 0.25};
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
-                        int qx = clampi(x + dx, 0, w - 1), qy = clampi(y + dy, 0, h - 1);
+                        int qx = clampi(x + dx * s, 0, w - 1), qy = clampi(y + dy * s, 0, h - 1);
                         gv += K3[dx + 1] * K3[dy + 1] * src[4 * ((size_t)qy * w + qx) + 3];
                     }
                 sigma = (double)p->phi_luma * sqrt(fmax(gv, 0.0) + (double)p->variance_eps);

@@ -386,6 +386,69 @@ def test_canonical_flat_edges_reduce_to_b3_blur_of_variance():
     np.testing.assert_allclose(out["denoised"][..., :3], col, rtol=1e-6)
 
 
+def brute_force_canonical_atrous(W0, G, p, iterations):
+    """Pure-Python loops over the CANONICAL a-trous iterations (written apart from the C
+    oracle): lattice 3x3 binomial variance prefilter, per-pixel depth gradient, depth /
+    normal / luminance edge stops, variance filtered by squared weights."""
+    h, w = len(W0), len(W0[0])
+    lum = lambda c: c[0] * LR + c[1] * LG + c[2] * LB  # noqa: E731
+    cl = lambda v, hi: min(max(v, 0), hi)  # noqa: E731
+    k3 = [0.25, 0.5, 0.25]
+    src = [[tuple(W0[y][x]) for x in range(w)] for y in range(h)]
+    hist = None
+    for k in range(iterations):
+        s = 1 << k
+        dst = [[None] * w for _ in range(h)]
+        for y in range(h):
+            for x in range(w):
+                c = src[y][x]
+                gv = sum(k3[i + 1] * k3[j + 1] * src[cl(y + j * s, h - 1)][cl(x + i * s, w - 1)][3]
+                         for j in (-1, 0, 1) for i in (-1, 0, 1))
+                sig = float(p.phi_luma) * math.sqrt(max(gv, 0.0) + float(p.variance_eps))
+                gz = lambda xx, yy: float(G[cl(yy, h - 1)][cl(xx, w - 1)][3])  # noqa: E731
+                gx = 0.5 * (gz(x + 1, y) - gz(x - 1, y))
+                gy = 0.5 * (gz(x, y + 1) - gz(x, y - 1))
+                gp = G[y][x]
+                acc = [0.0, 0.0, 0.0]
+                sw = sv = 0.0
+                for dy in range(-2, 3):
+                    for dx in range(-2, 3):
+                        qx, qy = cl(x + dx * s, w - 1), cl(y + dy * s, h - 1)
+                        q = src[qy][qx]
+                        gq = G[qy][qx]
+                        a = abs(lum(q) - lum(c)) / sig
+                        a += abs(float(gq[3]) - float(gp[3])) / (
+                            float(p.phi_depth) * abs(gx * dx * s + gy * dy * s) + float(p.depth_eps))
+                        d = sum(float(gp[t]) * float(gq[t]) for t in range(3))
+                        wn = d ** float(p.phi_normal) if d > 0 else 0.0
+                        wt = K5[dx + 2] * K5[dy + 2] * math.exp(-a) * wn
+                        for t in range(3):
+                            acc[t] += wt * q[t]
+                        sw += wt
+                        sv += wt * wt * q[3]
+                dst[y][x] = (acc[0] / sw, acc[1] / sw, acc[2] / sw, sv / (sw * sw))
+        if k == 0:
+            hist = dst
+        src = dst
+    return np.array(src), np.array(hist)
+
+
+def test_canonical_atrous_brute_force():
+    rng = np.random.default_rng(19)
+    h, w = 9, 11
+    L = rand_img(rng, h, w)
+    g = np.zeros((h, w, 4), np.float32)
+    n = rng.standard_normal((h, w, 3)) * 0.15 + np.array([0, 0, 1])
+    g[..., :3] = n / np.linalg.norm(n, axis=-1, keepdims=True)
+    g[..., 3] = 4 + rng.random((h, w)).astype(np.float32)
+    p = O.canonical_params(3)
+    p.phi_normal = 8.0  # weaker normal stop so every term matters
+    out = O.c_frame_f64(L, g, None, np.zeros((h, w, 2), np.float32), None, None, p, history_valid=False, debug=True)
+    want, hist = brute_force_canonical_atrous(out["variance"].tolist(), g.tolist(), p, 3)
+    np.testing.assert_allclose(out["denoised"][..., :3], want[..., :3], rtol=1e-11)
+    np.testing.assert_allclose(out["history"][..., :3], hist[..., :3], rtol=1e-11)
+
+
 def test_canonical_threads_do_not_change_result():
     rng = np.random.default_rng(18)
     h, w = 23, 31

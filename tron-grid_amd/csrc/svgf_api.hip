@@ -81,7 +81,6 @@ struct svgf_context {
     uint32_t w4 = 16;          // bytes of a wavelet-scratch pixel (FP32 in CANONICAL mode)
 
     void* scratch = nullptr;
-    PlaneDev wa{}, wb{}, nlen{};
 
     std::vector<StreamSlot> streams;
     uint64_t budget_used = 0;
@@ -165,16 +164,18 @@ uint64_t stream_state_bytes(uint32_t mode, uint32_t s4, uint32_t s2, uint32_t w,
 // (storage x4) and the uint8 history length.
 uint64_t frame_scratch_bytes(uint32_t s4, uint64_t px)
 {
-    return 2 * align_up(px * s4, ALIGN) + align_up(px, ALIGN);
+    return 2 * align_up(px * s4, ALIGN) + align_up(px, ALIGN) + align_up(px * 8, ALIGN);
 }
 
-void carve_scratch(char* base, uint32_t s4, uint32_t w, uint32_t h, PlaneDev& wa, PlaneDev& wb, PlaneDev& nlen)
+// wavelet ping-pong, history length (u8) and depth gradient (float2) of one frame
+void carve_scratch(char* base, uint32_t s4, uint32_t w, uint32_t h, FrameDesc& f)
 {
     uint64_t px = (uint64_t)w * h;
-    size_t p4 = align_up(px * s4, ALIGN);
-    wa = PlaneDev{base, (uint64_t)w * s4};
-    wb = PlaneDev{base + p4, (uint64_t)w * s4};
-    nlen = PlaneDev{base + 2 * p4, (uint64_t)w};
+    size_t p4 = align_up(px * s4, ALIGN), p1 = align_up(px, ALIGN);
+    f.wa = PlaneDev{base, (uint64_t)w * s4};
+    f.wb = PlaneDev{base + p4, (uint64_t)w * s4};
+    f.nlen = PlaneDev{base + 2 * p4, (uint64_t)w};
+    f.grad = PlaneDev{base + 2 * p4 + p1, (uint64_t)w * 8};
 }
 
 // Kernels address planes with 32-bit offsets through buffer resources.
@@ -352,7 +353,7 @@ svgf_status svgf_query_bytes(const svgf_desc* d, uint32_t w, uint32_t h, uint64_
         if (d->max_batch_frames) {
             uint64_t table = align_up(64 + 4ull * MAX_STAGES * STARTS_ROW, ALIGN) +
                              sizeof(FrameDesc) * (uint64_t)d->max_batch_frames;
-            uint64_t arena = d->max_batch_pixels * (2ull * w4 + 1) + (uint64_t)d->max_batch_frames * 3 * ALIGN;
+            uint64_t arena = d->max_batch_pixels * (2ull * w4 + 9) + (uint64_t)d->max_batch_frames * 4 * ALIGN;
             b += BATCH_SLOTS * (table + arena);
         }
         *context_bytes = b;
@@ -400,7 +401,6 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
         destroy_context(c);
         return SVGF_ERR_OUT_OF_MEMORY;
     }
-    carve_scratch(static_cast<char*>(c->scratch), c->w4,d->max_width, d->max_height, c->wa, c->wb, c->nlen);
     if (d->batch_flags & SVGF_PROFILE_STAGES) {
         for (auto& e : c->ev)
             if (hipEventCreate(&e) != hipSuccess) {
@@ -415,7 +415,7 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
         c->starts_off = 64;
         c->frames_off = align_up(c->starts_off + 4ull * MAX_STAGES * STARTS_ROW, ALIGN);
         c->table_bytes = c->frames_off + sizeof(FrameDesc) * (size_t)d->max_batch_frames;
-        c->arena_bytes = d->max_batch_pixels * (2ull * c->w4 + 1) + (uint64_t)d->max_batch_frames * 3 * ALIGN;
+        c->arena_bytes = d->max_batch_pixels * (2ull * c->w4 + 9) + (uint64_t)d->max_batch_frames * 4 * ALIGN;
         for (auto& s : c->slots) {
             if (hipHostMalloc(&s.staging, c->table_bytes, hipHostMallocDefault) != hipSuccess ||
                 hipMalloc(&s.table, c->table_bytes) != hipSuccess || hipMalloc(&s.arena, c->arena_bytes) != hipSuccess ||
@@ -497,7 +497,7 @@ svgf_status svgf_denoise(svgf_context* c, const svgf_frame* fr, void* hip_stream
         }
     }
     // Scratch planes are packed at this frame's width.
-    carve_scratch(static_cast<char*>(c->scratch), c->w4,w, h, f.wa, f.wb, f.nlen);
+    carve_scratch(static_cast<char*>(c->scratch), c->w4, w, h, f);
     hipStream_t st = static_cast<hipStream_t>(hip_stream);
     hipEvent_t* ev = profile_events(c);
     hipError_t e = c->mode == SVGF_MODE_REFERENCE ? launch_reference_frame(f, c->kp, c->st, st, ev)
@@ -679,7 +679,7 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
             f.hist_prev = s.hist[rd];
             f.hist_curr = s.hist[wr];
         }
-        carve_scratch(arena, c->w4, s.w, s.h, f.wa, f.wb, f.nlen);
+        carve_scratch(arena, c->w4, s.w, s.h, f);
         arena += frame_scratch_bytes(c->w4,(uint64_t)s.w * s.h);
         frames[i] = f;
     }

@@ -33,6 +33,7 @@ struct FrameDesc {
     PlaneDev den;        // output, storage width x4
     PlaneDev wa, wb;     // wavelet ping-pong scratch, storage width x4
     PlaneDev nlen;       // history length, uint8 (canonical)
+    PlaneDev grad;       // per-pixel depth gradient (gx, gy), float2 (canonical)
 };
 
 // Pass parameters as plain floats (svgf_params after validation).

@@ -306,6 +306,14 @@ __device__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Ti
     float l = luma_fma(R.x, R.y, R.z);
     float4 G = ldg4(gb, x, y);
     if (f.gsave.ptr) stg4(bind(f.gsave, f.h), x, y, G);
+    {
+        // depth gradient (central differences of clamped neighbours, halved), kept for the
+        // variance and a-trous stages of this frame
+        float zl = ldgz(gb, clampi(x - 1, 0, f.w - 1), y), zr = ldgz(gb, clampi(x + 1, 0, f.w - 1), y);
+        float zu = ldgz(gb, x, clampi(y - 1, 0, f.h - 1)), zd = ldgz(gb, x, clampi(y + 1, 0, f.h - 1));
+        const Buf gr = bind(f.grad, f.h);
+        st_f2(gr, y * gr.pitch + x * 8, make_float2(0.5f * (zr - zl), 0.5f * (zd - zu)));
+    }
 
     float sumb = 0.0f, h0 = 0.0f, h1 = 0.0f, h2 = 0.0f, hm1 = 0.0f, hm2 = 0.0f;
     // Compensated form: per-tap (weight, mean, second moment), weight 0 for unused taps so
@@ -439,11 +447,10 @@ __device__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Ti
     const Buf nl = bind(f.nlen, f.h);
     int n = (int)ld_u8(nl, y * nl.pitch + x);
     if (n >= 4) return;
-    const Buf gb = bind(f.g, f.h), wa = bind(f.wa, f.h);
+    const Buf gb = bind(f.g, f.h), wa = bind(f.wa, f.h), gr = bind(f.grad, f.h);
     float4 G = ldg4(gb, x, y);
-    float zl = ldgz(gb, clampi(x - 1, 0, f.w - 1), y), zr = ldgz(gb, clampi(x + 1, 0, f.w - 1), y);
-    float zu = ldgz(gb, x, clampi(y - 1, 0, f.h - 1)), zd = ldgz(gb, x, clampi(y + 1, 0, f.h - 1));
-    float gx = 0.5f * (zr - zl), gy = 0.5f * (zd - zu);
+    const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
+    const float gx = gxy.x, gy = gxy.y;
     float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy) {
@@ -502,37 +509,17 @@ __device__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile
     if (x <= max_x && y <= max_y) {
         const int c0 = (tj + LHALO) * LREG + ti + LHALO;
         const float4 C = sA[c0], Gc = sB[c0];
-        // 3x3 binomial of the variance and the depth gradient at pixel distance 1
-        float gv, gx, gy;
-        if (s == 1) {
-            gv = 0.0f;
+        // 3x3 binomial of the variance over the lattice neighbours (offsets +-s, in LDS) and the
+        // per-pixel depth gradient the temporal stage stored
+        float gv = 0.0f;
 #pragma unroll
-            for (int dy = -1; dy <= 1; ++dy)
+        for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-                for (int dx = -1; dx <= 1; ++dx)
-                    gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c0 + dy * LREG + dx], gv);
-            gx = 0.5f * (sB[c0 + 1].w - sB[c0 - 1].w);
-            gy = 0.5f * (sB[c0 + LREG].w - sB[c0 - LREG].w);
-        } else {
-            const int xl = max(x - 1, 0), xr = min(x + 1, max_x), yu = max(y - 1, 0), yd = min(y + 1, max_y);
-            const int xs[3] = {xl, x, xr}, ys[3] = {yu, y, yd};
-            gv = 0.0f;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    float v;
-                    if (a == 1 && b == 1) {
-                        v = sV[c0];
-                    } else {
-                        float sd = WavS::lda(src, xs[b], ys[a]);
-                        v = sd * sd;
-                    }
-                    gv = fmaf((b == 1 ? 0.5f : 0.25f) * (a == 1 ? 0.5f : 0.25f), v, gv);
-                }
-            gx = 0.5f * (ldgz(gb, xr, y) - ldgz(gb, xl, y));
-            gy = 0.5f * (ldgz(gb, x, yd) - ldgz(gb, x, yu));
-        }
+            for (int dx = -1; dx <= 1; ++dx)
+                gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c0 + dy * LREG + dx], gv);
+        const Buf gr = bind(f.grad, f.h);
+        const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
+        const float gx = gxy.x, gy = gxy.y;
         const float nls = -LOG2E / (p.phi_luma * __builtin_sqrtf(fmaxf(gv, 0.0f) + p.variance_eps));
         // -log2(e) / (phi_z |g.o| + eps) for the 12 offsets after the centre in row-major
         // order; the mirrored offset has the bitwise-same denominator.

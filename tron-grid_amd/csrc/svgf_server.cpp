@@ -103,6 +103,7 @@ void form_batch(svgf_server* s, std::vector<Pending>& out, const std::vector<uin
             if (out.size() + 1 > s->d.max_batch_frames || px + fpx > s->d.max_batch_pixels) break;
             px += fpx;
             out.push_back(*it);
+            seen.push_back(id);  // later frames of this stream wait for the next batch
             it = s->queue.erase(it);
             continue;
         }
