@@ -32,17 +32,19 @@ HBM_PEAK_GBPS = 8000.0
 # Algorithmic bytes per pixel, per stage, stage-structured rule: each stage reads each distinct
 # plane once and writes each output plane once at its storage width (extra taps count 0).
 #   temporal : radiance + G + motion + G_prev + history + moments read; wavelet W0 (colour,
-#              sigma) + moments + length (u8) written (+ the G copy kept for the next frame in
-#              the batched form)
+#              sigma; FP32) + moments + length (u8) + depth gradient (2 x FP32) written (+ the
+#              G copy kept for the next frame in the batched form)
 #   variance : length read; pixels with young history (n < 4) also read their 3x3 window and
 #              patch W0's sigma in place (data dependent, not counted)
-#   a-trous  : wavelet + G read; wavelet (or output) written; iteration 0 also reads the length
-#              and writes the colour history
+#   a-trous  : wavelet (FP32) + G + depth gradient read; wavelet written (the last iteration
+#              writes the output instead); iteration 0 also reads the length and writes the
+#              colour history
 def stage_bytes(storage_bytes4, storage_bytes2, iterations, batched):
-    s4, s2 = storage_bytes4, storage_bytes2
-    temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (s4 + s2 + 1) + (16 if batched else 0)
+    s4, s2, w4 = storage_bytes4, storage_bytes2, 16
+    temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (w4 + s2 + 1 + 8) + (16 if batched else 0)
     variance = 1
-    atrous = [(s4 + 16) + s4 + ((1 + s4) if k == 0 else 0) for k in range(iterations)]
+    atrous = [(w4 + 16 + 8) + (s4 if k == iterations - 1 else w4) + ((1 + s4) if k == 0 else 0)
+              for k in range(iterations)]
     return [temporal, variance] + atrous
 
 
@@ -396,6 +398,16 @@ def main():
                                        "what": "float restatement of svgf_denoise.slang (init + 3 passes), 1920x1080",
                                        "frames": reps}
 
+    traffic = args.traffic
+    if traffic is None:
+        # HBM bytes per a-trous launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+        # workload and recorded beside the other profile summaries
+        tfile = ROOT / "profiles" / "traffic.txt"
+        if tfile.exists() and args.frames == 64:
+            for ln in tfile.read_text().splitlines():
+                if ln.startswith("atrous_batched_fp16_mix64_bytes_per_launch:"):
+                    traffic = float(ln.split(":", 1)[1])
+
     ms_per_step = t_batched / args.steps * 1e3
     line = {
         "metric": "denoised Mpix/s + p99 latency, batched FP16 vs plain FP32 SVGF at <=1e-3 rel err",
@@ -439,7 +451,7 @@ def main():
         "stage_ms": {n: round(float(t), 5) for n, t in zip(names, stage_ms)},
         "roofline": {"bound": "hbm", "kernel": "k_batch<F16S, CANONICAL a-trous> (5 launches per batch)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": args.traffic,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": round(dom_bytes_px * batch_px),
                      "algorithmic_bytes_per_px": dom_bytes_px, "mean_launch_ms": round(dom_ms, 5),
                      "note": "arithmetic-bound: ~16 VALU + 2 transcendental ops per tap, 25 taps per pixel"},

@@ -109,7 +109,7 @@ svgf_status validate_params(const svgf_params& p, uint32_t mode)
     if (!finite_nonneg(p.phi_luma) || !finite_nonneg(p.alpha_moments) || p.alpha_moments > 1.0f) return SVGF_ERR_INVALID_ARGUMENT;
     if (!(p.variance_eps > 0.0f) || !finite_nonneg(p.variance_eps)) return SVGF_ERR_INVALID_ARGUMENT;
     if (mode == SVGF_MODE_CANONICAL) {
-        if (!finite_nonneg(p.phi_normal) || !finite_nonneg(p.phi_depth) || !finite_nonneg(p.alpha_color) ||
+        if (!(p.phi_luma > 0.0f) || !finite_nonneg(p.phi_normal) || !finite_nonneg(p.phi_depth) || !finite_nonneg(p.alpha_color) ||
             p.alpha_color > 1.0f || !(p.depth_eps > 0.0f) || !finite_nonneg(p.depth_reject) ||
             !(p.normal_reject >= -1.0f && p.normal_reject <= 1.0f) || p.history_cap < 1 || p.history_cap > 255)
             return SVGF_ERR_INVALID_ARGUMENT;

@@ -292,6 +292,13 @@ __device__ void ref_atrous_tile(const FrameDesc& f, const KParams& p, const Tile
 // FP16 round trip per iteration would add five half-ulp errors to the output.
 using WavS = F32S;
 
+// Canonical stages use the hardware reciprocal / square root / reciprocal square root
+// (v_rcp_f32, v_sqrt_f32, v_rsq_f32: about 1 ulp) instead of the correctly rounded
+// multi-instruction sequences; they are checked against the double oracle like the rest.
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // Temporal reprojection + colour / moments accumulation. Writes W0 = (colour, sqrt(temporal
 // variance)) into wavelet_a, the history length (uint8) and the moments; the batched form
 // also keeps G for the next frame. Standard deviations rather than variances are stored so
@@ -371,7 +378,7 @@ __device__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Ti
     float hmu = 0.0f, hvar = 0.0f;
     if (sumb > 0.0f) {
         n_new = min(nmax + 1, p.history_cap);
-        float inv = 1.0f / sumb;
+        float inv = rcp_hw(sumb);
         h0 *= inv;
         h1 *= inv;
         h2 *= inv;
@@ -396,7 +403,7 @@ __device__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Ti
         h0 = h1 = h2 = 0.0f;
         hm1 = hm2 = 0.0f;
     }
-    float rn = 1.0f / (float)n_new;
+    float rn = rcp_hw((float)n_new);
     float ac = fmaxf(p.alpha_color, rn), am = fmaxf(p.alpha_moments, rn);
     float c0 = fmaf(ac, R.x - h0, h0), c1 = fmaf(ac, R.y - h1, h1), c2 = fmaf(ac, R.z - h2, h2);
     float var;
@@ -405,7 +412,7 @@ __device__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Ti
         float d = l - hmu;
         float mu = fmaf(am, d, hmu);
         float m2 = (1.0f - am) * fmaf(am * d, d, hvar);
-        S::st2(mc, x, y, make_float2(mu, __builtin_sqrtf(m2)));
+        S::st2(mc, x, y, make_float2(mu, sqrt_hw(m2)));
         var = m2;
     } else {
         float mu1 = fmaf(am, l - hm1, hm1);
@@ -413,7 +420,7 @@ __device__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Ti
         S::st2(mc, x, y, make_float2(mu1, mu2));
         var = fmaxf(mu2 - mu1 * mu1, 0.0f);
     }
-    WavS::st4(bind(f.wa, f.h), x, y, make_float4(c0, c1, c2, __builtin_sqrtf(var)));
+    WavS::st4(bind(f.wa, f.h), x, y, make_float4(c0, c1, c2, sqrt_hw(var)));
     const Buf nl = bind(f.nlen, f.h);
     st_u8(nl, y * nl.pitch + x, (uint32_t)n_new);
 }
@@ -442,11 +449,27 @@ __device__ __forceinline__ float edge_log2(const KParams& p, float4 Gc, float4 G
 template <class S>
 __device__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
-    const int x = T.i0 + (int)(threadIdx.x & (LT - 1)), y = T.j0 + (int)(threadIdx.x >> 4);
-    if (x >= f.w || y >= f.h) return;
-    const Buf nl = bind(f.nlen, f.h);
-    int n = (int)ld_u8(nl, y * nl.pitch + x);
-    if (n >= 4) return;
+    // Young pixels are scattered (disocclusions), so nearly every wave holds a few of them.
+    // Compact them first: the tile's young pixels go to the leading lanes of the block, and
+    // only ceil(count / 64) waves run the 3x3 window.
+    __shared__ int sList[BLOCK];
+    __shared__ int sCount;
+    const int tid = (int)threadIdx.x;
+    {
+        const int x = T.i0 + (tid & (LT - 1)), y = T.j0 + (tid >> 4);
+        bool young = false;
+        if (x < f.w && y < f.h) {
+            const Buf nl = bind(f.nlen, f.h);
+            young = ld_u8(nl, y * nl.pitch + x) < 4u;
+        }
+        if (tid == 0) sCount = 0;
+        __syncthreads();
+        if (young) sList[atomicAdd(&sCount, 1)] = tid;
+        __syncthreads();
+    }
+    if (tid >= sCount) return;
+    const int src_lane = sList[tid];
+    const int x = T.i0 + (src_lane & (LT - 1)), y = T.j0 + (src_lane >> 4);
     const Buf gb = bind(f.g, f.h), wa = bind(f.wa, f.h), gr = bind(f.grad, f.h);
     float4 G = ldg4(gb, x, y);
     const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
@@ -470,7 +493,7 @@ __device__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Ti
     }
     float var = 0.0f;
     if (sw > 0.0f) {
-        float inv = 1.0f / sw;
+        float inv = rcp_hw(sw);
         float mean = s1 * inv, s2 = 0.0f;
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
@@ -479,7 +502,7 @@ __device__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Ti
         }
         var = s2 * inv;
     }
-    WavS::sta(wa, x, y, __builtin_sqrtf(var));
+    WavS::sta(wa, x, y, sqrt_hw(var));
 }
 
 // A-trous iteration k (stride s = 2^k) on one lattice tile: luminance, depth and normal
@@ -520,7 +543,7 @@ __device__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile
         const Buf gr = bind(f.grad, f.h);
         const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
         const float gx = gxy.x, gy = gxy.y;
-        const float nls = -LOG2E / (p.phi_luma * __builtin_sqrtf(fmaxf(gv, 0.0f) + p.variance_eps));
+        const float nls = (-LOG2E / p.phi_luma) * rsq_hw(fmaxf(gv, 0.0f) + p.variance_eps);
         // -log2(e) / (phi_z |g.o| + eps) for the 12 offsets after the centre in row-major
         // order; the mirrored offset has the bitwise-same denominator.
         float nrd[12];
@@ -553,10 +576,10 @@ __device__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile
         }
         float4 o;
         if (sw > 1e-6f) {
-            float inv = 1.0f / sw;
-            o = make_float4(s0 * inv, s1 * inv, s2 * inv, __builtin_sqrtf(sv * inv * inv));
+            float inv = rcp_hw(sw);
+            o = make_float4(s0 * inv, s1 * inv, s2 * inv, sqrt_hw(sv) * inv);
         } else {
-            o = make_float4(C.x, C.y, C.z, __builtin_sqrtf(sV[c0]));
+            o = make_float4(C.x, C.y, C.z, sqrt_hw(sV[c0]));
         }
         if (k == 0) {
             const Buf nl = bind(f.nlen, f.h);
