@@ -29,7 +29,7 @@ def seq_inputs(w, h, frames, *, seed0=0, stream=0, noise=0, pan_px=0.5, disocclu
     return out
 
 
-def oracle_canonical_sequence(inputs, params):
+def oracle_canonical_sequence(inputs, params, threads=1):
     """Run the FP64 canonical oracle over a sequence; returns per-frame output dicts."""
     import oracle as O
 
@@ -38,7 +38,7 @@ def oracle_canonical_sequence(inputs, params):
     outs = []
     for f, d in enumerate(inputs):
         o = O.c_frame_f64(d["radiance"], d["gbuffer"], gprev, d["motion"], hist, mom, params,
-                          history_valid=f > 0, debug=True)
+                          history_valid=f > 0, debug=True, threads=threads)
         hist, mom, gprev = o["history"], o["moments"], d["gbuffer"]
         outs.append(o)
     return outs
@@ -69,3 +69,26 @@ def gpu_canonical_sequence(ctx, inputs, dtype):
         outs.append({"denoised": den.float().cpu().numpy(), "history": hist[cur].float().cpu().numpy(),
                      "moments": mom[cur].float().cpu().numpy()})
     return outs
+
+
+# Receptive field of a CANONICAL output after 3 frames: 5 a-trous passes reach 2s each
+# (taps and the variance prefilter are both centred on the pixel; s = 1..16) = 62, the depth
+# gradient and the spatial variance 1 each, the reprojection 2, and each earlier frame's
+# history (reprojection 2 + variance 1 + first pass 2) 5 more: under 80 in all. On the
+# 640x520 oracle sequence of the locality test the farthest pixel a crop edge changes is 68.
+MARGIN = 128
+
+
+def crop_windows(w, h, c):
+    """(y0, x0) of c x c crops at the top-left corner, the centre and the bottom-right corner."""
+    return [(0, 0), ((h - c) // 2, (w - c) // 2), (h - c, w - c)]
+
+
+def crop_interior(y0, x0, c, w, h):
+    """Slices of a crop that are beyond the receptive field of every crop edge that is not an
+    image edge (clamping and reprojection rejection agree there)."""
+    top = 0 if y0 == 0 else MARGIN
+    left = 0 if x0 == 0 else MARGIN
+    bottom = c if y0 + c == h else c - MARGIN
+    right = c if x0 + c == w else c - MARGIN
+    return slice(top, bottom), slice(left, right)

@@ -1,0 +1,127 @@
+"""GPU tests at the workload sizes: 1920x1080 sequences against the oracle directly, and
+size-independent properties at 3840x2160 (where the FP64 oracle would take minutes).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from svgf_testutil import crop_interior, crop_windows, gpu_canonical_sequence, oracle_canonical_sequence, rel_err, seq_inputs
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def S():
+    import svgf_amd
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    svgf_amd.lib()
+    return svgf_amd
+
+
+def test_reference_fp32_1080p_16_frames(S, oracle_lib):
+    """1920x1080, 16 frames with per-frame seeds and 0.02 m/frame pan, moments chained through
+    the caller-owned ping-pong: every frame within 1e-5 of the float restatement."""
+    O = oracle_lib
+    w, h = 1920, 1080
+    pan = 0.02 * (h / (2.0 * np.tan(np.pi / 6))) / 10.0
+    dev = torch.device("cuda")
+    mom = [torch.zeros((h, w, 2), device=dev) for _ in range(2)]
+    den = torch.empty((h, w, 4), device=dev)
+    om = None
+    worst = 0.0
+    with S.Context(S.MODE_REFERENCE, S.STORAGE_FP32, max_width=w, max_height=h) as ctx:
+        for f in range(16):
+            L = S.synth_frame(w, h, seed=f, frame=f, pan_px=pan, disocclusion=0.1)["radiance"]
+            ctx.denoise(radiance=torch.from_numpy(L).to(dev), denoised=den, moments_curr=mom[f % 2],
+                        moments_prev=mom[(f + 1) % 2] if f else None, history_valid=f > 0)
+            torch.cuda.synchronize()
+            om, od, _, _ = O.r_frame_f32(L, om, O.reference_params(3), history_valid=f > 0, threads=8)
+            d_max, _ = rel_err(den.cpu().numpy(), od)
+            m_max, _ = rel_err(mom[f % 2].cpu().numpy(), om)
+            worst = max(worst, d_max, m_max)
+            assert d_max <= 1e-5 and m_max <= 1e-5, (f, d_max, m_max)
+    print(f"1080p x 16 REFERENCE FP32: max rel err {worst:.3e}")
+
+
+def test_reference_fp16_720p_half_store(S, oracle_lib):
+    """The reference's own published case: 1280x720, rgba16f / rg16f, init + 3 passes."""
+    O = oracle_lib
+    w, h = 1280, 720
+    dev = torch.device("cuda")
+    L0 = O.round_f16(S.synth_frame(w, h, seed=1, frame=0)["radiance"])
+    L1 = O.round_f16(S.synth_frame(w, h, seed=0, frame=1)["radiance"])
+    mom = [torch.zeros((h, w, 2), dtype=torch.float16, device=dev) for _ in range(2)]
+    den = torch.empty((h, w, 4), dtype=torch.float16, device=dev)
+    with S.Context(S.MODE_REFERENCE, S.STORAGE_FP16, max_width=w, max_height=h) as ctx:
+        ctx.denoise(radiance=torch.from_numpy(L0).to(dev).half(), denoised=den, moments_curr=mom[0],
+                    history_valid=False)
+        ctx.denoise(radiance=torch.from_numpy(L1).to(dev).half(), denoised=den, moments_curr=mom[1],
+                    moments_prev=mom[0])
+        torch.cuda.synchronize()
+    om0, _, _, _ = O.r_frame_f32(L0, None, O.reference_params(3), store=O.STORE_F16, history_valid=False, threads=8)
+    om1, od1, _, _ = O.r_frame_f32(L1, om0, O.reference_params(3), store=O.STORE_F16, threads=8)
+    d_max, d_mean = rel_err(den.float().cpu().numpy(), od1)
+    assert d_max <= 2.0 ** -10 + 1e-6 and d_mean <= 1e-5
+    assert np.array_equal(mom[1].float().cpu().numpy(), om1)
+
+
+@pytest.mark.parametrize("form", ["fp32", "fp16_compensated"])
+def test_canonical_4k_windows_vs_oracle(S, oracle_lib, form):
+    """3840x2160, 3 frames, fractional pan, 10% disocclusion: the full-frame GPU result agrees
+    with the FP64 oracle run on 448x448 crops of the same inputs, inside each crop's interior
+    (corner, centre and opposite corner), to the same bar as the small sequences."""
+    O = oracle_lib
+    w, h, c = 3840, 2160, 448
+    fp16 = form != "fp32"
+    inputs = seq_inputs(w, h, 3, noise=1 if fp16 else 0, pan_px=1.37, disocclusion=0.1, half=fp16)
+    op = O.canonical_params(5, compensated=fp16)
+    storage, dt = (S.STORAGE_FP16, torch.float16) if fp16 else (S.STORAGE_FP32, torch.float32)
+    with S.Context(S.MODE_CANONICAL, storage, max_width=w, max_height=h) as ctx:
+        got = gpu_canonical_sequence(ctx, inputs, dt)[-1]
+    for y0, x0 in crop_windows(w, h, c):
+        crop = [{k: np.ascontiguousarray(v[y0:y0 + c, x0:x0 + c]) for k, v in d.items()} for d in inputs]
+        ref = oracle_canonical_sequence(crop, op)[-1]
+        ys, xs = crop_interior(y0, x0, c, w, h)
+        g = {k: v[y0:y0 + c, x0:x0 + c][ys, xs] for k, v in got.items()}
+        r = {k: v[ys, xs] for k, v in ref.items()}
+        dm, dmean = rel_err(g["denoised"][..., :3], r["denoised"][..., :3])
+        print(f"{form} window ({y0},{x0}): denoised max {dm:.3e} mean {dmean:.3e}")
+        assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), (y0, x0)
+        if fp16:
+            assert np.isfinite(g["denoised"]).all() and dmean <= 1e-3, (y0, x0)
+        else:
+            assert dm <= 2e-4 and dmean <= 1e-6, (y0, x0)
+
+
+def test_canonical_4k_constant_image_is_fixed_point(S):
+    """A constant radiance over a flat G-buffer stays constant through every stage (the
+    weights are normalised), in both storage forms."""
+    w, h = 3840, 2160
+    dev = torch.device("cuda")
+    col = np.array([2.5, 1.25, 0.75, 1.0], np.float32)
+    for storage, dt in ((S.STORAGE_FP32, torch.float32), (S.STORAGE_FP16, torch.float16)):
+        rad = torch.from_numpy(np.broadcast_to(col, (h, w, 4)).copy()).to(dev).to(dt)
+        g = torch.zeros((h, w, 4), device=dev)
+        g[..., 2] = 1.0
+        g[..., 3] = 7.0
+        mv = torch.zeros((h, w, 2), device=dev)
+        hist = [torch.zeros((h, w, 4), dtype=dt, device=dev) for _ in range(2)]
+        mom = [torch.zeros((h, w, 2), dtype=dt, device=dev) for _ in range(2)]
+        den = torch.empty((h, w, 4), dtype=dt, device=dev)
+        with S.Context(S.MODE_CANONICAL, storage, max_width=w, max_height=h) as ctx:
+            for f in range(3):
+                ctx.denoise(radiance=rad, gbuffer=g, gbuffer_prev=g if f else None, motion=mv,
+                            history_prev=hist[(f + 1) % 2] if f else None, history_curr=hist[f % 2],
+                            moments_prev=mom[(f + 1) % 2] if f else None, moments_curr=mom[f % 2], denoised=den,
+                            history_valid=f > 0)
+            torch.cuda.synchronize()
+        o = den.float().cpu().numpy()
+        mx, _ = rel_err(o[..., :3], np.broadcast_to(col[:3], (h, w, 3)))
+        assert mx <= 1e-5, (storage, mx)
+        assert np.all(o[..., 3] == 1.0)
+        n = hist[0].float().cpu().numpy()[..., 3]  # frame 2 wrote parity 0
+        assert np.all(n == 3.0)

@@ -481,3 +481,29 @@ def test_canonical_spatial_variance_is_window_variance():
         for x in range(w):
             win = [lum[min(max(y + dy, 0), h - 1), min(max(x + dx, 0), w - 1)] for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
             np.testing.assert_allclose(out["variance"][y, x, 3], np.var(win), rtol=1e-10, atol=1e-14)
+
+
+@pytest.mark.parametrize("compensated", [False, True])
+def test_canonical_crop_interior_equals_full_frame(compensated):
+    """Locality of the CANONICAL pass: over 3 frames with fractional pan and disocclusion, the
+    oracle on a 448x448 crop reproduces the full-frame oracle exactly inside the crop's
+    interior (MARGIN from every edge that is not an image edge). This pins the margin that the
+    full-size GPU window test relies on; a margin below the receptive field fails here."""
+    import svgf_amd as S
+    from svgf_testutil import MARGIN, crop_interior, crop_windows, oracle_canonical_sequence, seq_inputs
+
+    S.lib()
+    w, h, c = 640, 520, 448
+    inputs = seq_inputs(w, h, 3, noise=1, pan_px=1.37, disocclusion=0.1)
+    op = O.canonical_params(5, compensated=compensated)
+    full = oracle_canonical_sequence(inputs, op, threads=8)[-1]
+    for y0, x0 in crop_windows(w, h, c):
+        crop = [{k: np.ascontiguousarray(v[y0:y0 + c, x0:x0 + c]) for k, v in d.items()} for d in inputs]
+        ref = oracle_canonical_sequence(crop, op, threads=8)[-1]
+        ys, xs = crop_interior(y0, x0, c, w, h)
+        for k in ("denoised", "history", "moments"):
+            assert np.array_equal(full[k][y0:y0 + c, x0:x0 + c][ys, xs], ref[k][ys, xs]), (k, y0, x0)
+        # the margin is needed: just outside the interior the crop edge changes the result
+        if x0 > 0:
+            a = full["denoised"][y0:y0 + c, x0:x0 + c][ys, :MARGIN // 4]
+            assert not np.array_equal(a, ref["denoised"][ys, :MARGIN // 4])
