@@ -146,6 +146,9 @@ def main():
                          "copy: one 1 GiB device copy (counter calibration)")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched form (for profiler runs)")
+    ap.add_argument("--launch", choices=["persistent", "per_stage"], default="per_stage",
+                    help="batched launch form of the headline value: one launch per stage (each over the tiles of "
+                         "all frames), or one persistent launch per batch (the other form is timed beside it)")
     ap.add_argument("--report-frames", type=int, default=16)
     ap.add_argument("--report-scale", type=int, default=1)
     ap.add_argument("--report-threads", type=int, default=16)
@@ -210,18 +213,15 @@ def main():
     torch.cuda.synchronize()
     gen_s = time.time() - t0
 
-    # ---- batched FP16 canonical form ----
+    # ---- batched FP16 canonical form: per-stage launches and one persistent launch per batch ----
     p16 = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
-    ctx16 = S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p16, max_width=maxw, max_height=maxh,
-                      max_streams=args.frames, max_batch_frames=args.frames, max_batch_pixels=batch_px)
-    sids = [ctx16.stream_open(w, h) for w, h in shapes]
     out16 = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in shapes]
     reqs = []
     for f in range(ND):
         arr = (S.Request * args.frames)()
         for i, (w, h) in enumerate(shapes):
             r = arr[i]
-            r.stream_id = sids[i]
+            r.stream_id = i  # every batched context below opens its streams in this order
             r.radiance = S.plane_of(rad16[i][f], S.FMT_RGBA16F)
             r.gbuffer = S.plane_of(gbuf[i][f], S.FMT_RGBA32F)
             r.motion = S.plane_of(mvec[i][f], S.FMT_RG32F)
@@ -230,28 +230,50 @@ def main():
     L = S.lib()
     sh = ctypes.c_void_p(stream.cuda_stream)
 
+    def batched_context(flags):
+        ctx = S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p16, max_width=maxw, max_height=maxh,
+                        max_streams=args.frames, max_batch_frames=args.frames, max_batch_pixels=batch_px,
+                        batch_flags=flags)
+        ids = [ctx.stream_open(w, h) for w, h in shapes]
+        assert ids == list(range(len(shapes)))
+        return ctx
+
     def batch_step(ctx, k):
         S.check(L.svgf_denoise_batch(ctx.handle, reqs[k % ND], args.frames, sh), "svgf_denoise_batch")
 
-    with torch.cuda.stream(stream):
-        for k in range(args.warmup):
-            batch_step(ctx16, k)
-        torch.cuda.synchronize()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            evs[k][0].record(stream)
-            batch_step(ctx16, args.warmup + k)
-            evs[k][1].record(stream)
-        torch.cuda.synchronize()
-        t_batched = time.perf_counter() - t0
-    lat = np.array([a.elapsed_time(b) for a, b in evs])
+    def time_batched(flags):
+        ctx = batched_context(flags)
+        with torch.cuda.stream(stream):
+            for k in range(args.warmup):
+                batch_step(ctx, k)
+            torch.cuda.synchronize()
+            evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                   for _ in range(args.steps)]
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                evs[k][0].record(stream)
+                batch_step(ctx, args.warmup + k)
+                evs[k][1].record(stream)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+        ctx.close()
+        return t, np.array([a.elapsed_time(b) for a, b in evs])
+
+    forms = {"persistent": S.BATCH_PERSISTENT, "per_stage": 0}
+    head = args.launch
+    t_batched, lat = time_batched(forms[head])
     batched_mpx = args.steps * batch_px / t_batched / 1e6
+    other = None
+    if not args.batched_only:
+        name = "per_stage" if head == "persistent" else "persistent"
+        t_o, lat_o = time_batched(forms[name])
+        other = {"form": name, "value": round(args.steps * batch_px / t_o / 1e6, 2), "unit": "Mpx/s",
+                 "ms_per_step": round(t_o / args.steps * 1e3, 4),
+                 "p99_ms": round(float(np.percentile(lat_o, 99)), 4)}
 
     if args.batched_only:
-        print(json.dumps({"batched_fp16_mpx_per_s": round(batched_mpx, 2), "steps": args.steps,
+        print(json.dumps({"batched_fp16_mpx_per_s": round(batched_mpx, 2), "launch": head, "steps": args.steps,
                           "ms_per_step": round(t_batched / args.steps * 1e3, 4), "pixels_per_batch": batch_px}))
-        ctx16.close()
         return
 
     # ---- plain FP32 canonical form on the same mix: one frame per launch sequence, no graph ----
@@ -294,11 +316,7 @@ def main():
     plain_mpx = args.steps * batch_px / t_plain / 1e6
 
     # ---- profiled pass: per-stage kernel time (hipEvents around every stage launch) ----
-    ctxp = S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p16, max_width=maxw, max_height=maxh,
-                     max_streams=args.frames, max_batch_frames=args.frames, max_batch_pixels=batch_px,
-                     batch_flags=S.PROFILE_STAGES)
-    psids = [ctxp.stream_open(w, h) for w, h in shapes]
-    assert psids == sids
+    ctxp = batched_context(S.PROFILE_STAGES)
     stage_ms = []
     with torch.cuda.stream(stream):
         for k in range(args.warmup + args.steps):
@@ -363,7 +381,8 @@ def main():
         budget = 64 << 30
         sctx = S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p16, max_width=1280, max_height=720,
                          max_streams=args.serve_streams + 64, max_batch_frames=args.frames,
-                         max_batch_pixels=max(batch_px, 1280 * 720 * 8), hbm_budget_bytes=budget)
+                         max_batch_pixels=max(batch_px, 1280 * 720 * 8), hbm_budget_bytes=budget,
+                         batch_flags=forms[head])
         with S.Server(sctx, lanes=2) as srv:
             serving = srv.simulate(streams=args.serve_streams, classes=classes, class_inputs=class_inputs,
                                    rate_hz=args.serve_rate, duration_s=args.serve_seconds, churn_per_s=0.05, seed=11)
@@ -430,7 +449,10 @@ def main():
         "dtype": "fp16-storage/fp32-math",
         "data": f"synthetic seeded G-buffers ({ND} distinct frames per stream, cycled)",
         "config": {
-            "workload": "batched canonical SVGF, 64 agent frames per launch sequence, ragged tiles, hipGraph",
+            "workload": ("batched canonical SVGF, 64 agent frames per persistent launch (ragged tile tickets), hipGraph"
+                         if head == "persistent" else
+                         "batched canonical SVGF, 64 agent frames per launch sequence (one launch per stage), hipGraph"),
+            "launch": head,
             "frames_per_batch": args.frames,
             "pixels_per_batch": batch_px,
             "resolutions": "uniform over 96x96,128x128,256x256,320x180,640x480,1280x720 (seed 0)",
@@ -445,13 +467,14 @@ def main():
         "reference_mode_720p": ref720,
         "serving": None if serving is None else {
             "what": f"{args.serve_streams} agent streams, Poisson {args.serve_rate:g} req/s, 5%/s churn (cold "
-                    f"history), 64 GB HBM budget, 2 in-flight batches, {args.serve_seconds:g} s",
+                    f"history), 64 GB HBM budget, 2 in-flight batches, {args.serve_seconds:g} s, {head} launches",
             "p50_ms": round(serving["latency_p50_ms"], 4), "p99_ms": round(serving["latency_p99_ms"], 4),
             "max_ms": round(serving["latency_max_ms"], 4), "mpx_per_s": round(serving["mpx_per_s"], 2),
             "offered_mpx_per_s": round(serving["offered_mpx_per_s"], 2),
             "requests": int(serving["completed"]), "rejected": int(serving["rejected"]),
             "mean_batch_frames": round(serving["mean_batch_frames"], 2),
             "streams_opened": int(serving["streams_opened"])},
+        "batched_other_launch": other,
         "plain_fp32": {"value": round(plain_mpx, 2), "unit": "Mpx/s", "bytes_per_px": bytes_px_32,
                        "ratio_batched_over_plain": round(batched_mpx / plain_mpx, 3)},
         "effective_GBps": round(batched_mpx * 1e6 * bytes_px_16 / 1e9, 1),
@@ -466,7 +489,6 @@ def main():
         "input_generation_s": round(gen_s, 2),
     }
     print(json.dumps(line))
-    ctx16.close()
     ctx32.close()
     ctxp.close()
 

@@ -109,8 +109,12 @@ typedef struct svgf_desc {
 /* Flags of svgf_desc.batch_flags */
 #define SVGF_BATCH_NO_GRAPH 0x1u /* launch the batched stages directly instead of replaying a
                                     captured hipGraph */
-#define SVGF_PROFILE_STAGES 0x2u /* record a hipEvent around every stage launch (no graph);
-                                    read back with svgf_stage_times */
+#define SVGF_PROFILE_STAGES 0x2u /* record a hipEvent around every stage launch (no graph,
+                                    one launch per stage); read back with svgf_stage_times */
+#define SVGF_BATCH_PERSISTENT 0x4u /* run all stages of a batch in one persistent launch: the
+                                      workgroups pull (stage, frame, tile) tickets in dependency
+                                      order and wait per frame for the previous stage, instead
+                                      of one launch per stage */
 
 typedef struct svgf_plane {
     void* ptr;            /* device pointer to row 0, pixel 0 */
@@ -195,7 +199,11 @@ SVGF_API svgf_status svgf_stream_state_get(svgf_context* ctx, uint32_t stream_id
 /* Bytes of the HBM budget in use by open streams. */
 SVGF_API svgf_status svgf_budget_used(svgf_context* ctx, uint64_t* bytes);
 
-/* --- batched form: n requests of uneven sizes in one ragged-tile launch sequence -------- */
+/* --- batched form: n requests of uneven sizes in one ragged-tile launch sequence -------- *
+ * With SVGF_BATCH_PERSISTENT, a batch is one kernel launch; its dependency waits are bounded
+ * (250 ms each; the environment variable SVGF_PERSISTENT_WAIT_US, read by svgf_create, sets
+ * another bound). A batch whose wait gave up has wrong outputs: the next call that reuses its
+ * internal slot (every third call) returns SVGF_ERR_TIMEOUT and enqueues nothing. */
 SVGF_API svgf_status svgf_denoise_batch(svgf_context* ctx, const svgf_request* requests, uint32_t n,
                                void* hip_stream);
 

@@ -4,6 +4,7 @@ behave. No kernel is launched here."""
 from __future__ import annotations
 
 import ctypes
+import pathlib
 import subprocess
 
 import numpy as np
@@ -88,6 +89,20 @@ def test_create_rejects_bad_descriptors():
     assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == 1
     bad = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, S.default_params(0), 64, 64, 0, 0, 0, 0, 0, 0x80)
     assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == 1
+    # every declared batch flag passes validation (here it then stops at the device check)
+    known = S.BATCH_NO_GRAPH | S.PROFILE_STAGES | S.BATCH_PERSISTENT
+    ok = S.Desc(S.MODE_CANONICAL, S.STORAGE_FP16, S.default_params(1, 1), 64, 64, 4, 4, 4096, 0, 0, known)
+    try:
+        import torch
+
+        gpu = torch.cuda.is_available()
+    except ImportError:
+        gpu = False
+    rc = L.svgf_create(ctypes.byref(ok), ctypes.byref(h))
+    if gpu:
+        assert rc == 0 and L.svgf_destroy(h) == 0
+    else:
+        assert rc == 8
     p = S.default_params(0)
     p.iterations = 0
     bad = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, p, 64, 64, 0, 0, 0, 0, 0, 0)
@@ -161,3 +176,15 @@ def test_synth_disocclusion_fraction():
     base = S.synth_frame(400, 300, seed=0, frame=2, pan_px=0.0, disocclusion=0.0)
     frac = np.mean(d["gbuffer"][..., 3] != base["gbuffer"][..., 3])
     assert 0.09 < frac < 0.11
+
+
+def test_binding_flags_match_header():
+    """The ctypes binding's flag constants are the values include/svgf.h defines."""
+    import re
+
+    text = (pathlib.Path(__file__).resolve().parent.parent / "include" / "svgf.h").read_text()
+    defs = {m.group(1): int(m.group(2), 16) for m in re.finditer(r"#define SVGF_(\w+) (0x[0-9a-fA-F]+)u", text)}
+    assert defs["BATCH_NO_GRAPH"] == S.BATCH_NO_GRAPH
+    assert defs["PROFILE_STAGES"] == S.PROFILE_STAGES
+    assert defs["BATCH_PERSISTENT"] == S.BATCH_PERSISTENT
+    assert defs["FLAG_COMPENSATED_MOMENTS"] == S.FLAG_COMPENSATED

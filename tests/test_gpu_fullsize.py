@@ -125,3 +125,50 @@ def test_canonical_4k_constant_image_is_fixed_point(S):
         assert np.all(o[..., 3] == 1.0)
         n = hist[0].float().cpu().numpy()[..., 3]  # frame 2 wrote parity 0
         assert np.all(n == 3.0)
+
+
+def test_persistent_matches_per_stage_at_scale(S):
+    """The persistent launch (every stage of a batch in one kernel, ragged tiles pulled as
+    tickets) gives the same bits as one launch per stage, on a 24-frame mix with 1280x720
+    frames (about 6 Mpx, 170k tickets per batch), over 3 batches: outputs and the resident
+    history / moments / G copy. Small batches did not show a wave of a workgroup reading the
+    next ticket before thread 0's LDS write had landed; at this size every stream differed."""
+    dev = torch.device("cuda")
+    res = [(96, 96), (128, 128), (256, 256), (320, 180), (640, 480), (1280, 720)]
+    rng = np.random.default_rng(3)
+    shapes = [res[i] for i in rng.integers(0, len(res), 24)]
+    npx = sum(w * h for w, h in shapes)
+    ins = []
+    for i, (w, h) in enumerate(shapes):
+        fr = []
+        for f in range(2):
+            d = S.synth_frame(w, h, seed=1000 * i + f, stream=i, frame=f, pan_px=0.5, disocclusion=0.05)
+            fr.append({"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                       "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                       "motion": torch.from_numpy(d["motion"]).to(dev)})
+        ins.append(fr)
+
+    def context(flags):
+        c = S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=1280, max_height=720, max_streams=len(shapes),
+                      max_batch_frames=len(shapes), max_batch_pixels=npx, batch_flags=flags)
+        return c, [c.stream_open(w, h) for w, h in shapes]
+
+    (A, ia), (B, ib) = context(0), context(S.BATCH_PERSISTENT)
+    try:
+        outA = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in shapes]
+        outB = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in shapes]
+        for k in range(3):
+            for c, ids, outs in ((A, ia, outA), (B, ib, outB)):
+                c.denoise_batch([{"stream_id": ids[i], "denoised": outs[i], **ins[i][k % 2]} for i in range(len(shapes))])
+            torch.cuda.synchronize()
+            for i, (w, h) in enumerate(shapes):
+                assert torch.equal(outA[i], outB[i]), (k, i, w, h)
+                sa, sb = A.stream_state(ia[i]), B.stream_state(ib[i])
+                for name, ch, dt in (("history", 4, torch.float16), ("moments", 2, torch.float16),
+                                     ("gbuffer", 4, torch.float32)):
+                    ta = S.device_tensor_from_plane(getattr(sa, name), h, w, ch, dt)
+                    tb = S.device_tensor_from_plane(getattr(sb, name), h, w, ch, dt)
+                    assert torch.equal(ta, tb), (k, i, name)
+    finally:
+        A.close()
+        B.close()

@@ -131,21 +131,24 @@ def _ragged_shapes():
 
 
 @pytest.mark.parametrize("mode", ["reference", "canonical"])
-@pytest.mark.parametrize("graph", [True, False])
-def test_batch_matches_single(S, mode, graph):
-    """Each stream's result is bit-identical alone (single-frame form) and in a batch."""
+@pytest.mark.parametrize("launch", ["graph", "direct", "persistent_graph", "persistent_direct"])
+def test_batch_matches_single(S, mode, launch):
+    """Each stream's result is bit-identical alone (single-frame form) and in a batch, for the
+    per-stage launches and the persistent launch, replayed from a graph or launched directly.
+    5 batches on 3 internal slots: the persistent form reuses a slot's counters."""
     m = S.MODE_REFERENCE if mode == "reference" else S.MODE_CANONICAL
     st = S.STORAGE_FP16
     dt = torch.float16
     dev = torch.device("cuda")
     shapes = _ragged_shapes()
-    frames = 3
+    frames = 5
     inputs = [seq_inputs(w, h, frames, seed0=10 * i, stream=i, pan_px=0.7, disocclusion=0.1) for i, (w, h) in
               enumerate(shapes)]
     maxw = max(w for w, _ in shapes)
     maxh = max(h for _, h in shapes)
     npx = sum(w * h for w, h in shapes)
-    flags = 0 if graph else S.BATCH_NO_GRAPH
+    flags = (S.BATCH_PERSISTENT if launch.startswith("persistent") else 0) | \
+        (S.BATCH_NO_GRAPH if launch.endswith("direct") else 0)
     with S.Context(m, st, max_width=maxw, max_height=maxh, max_streams=8, max_batch_frames=8,
                    max_batch_pixels=npx, batch_flags=flags) as bctx:
         sids = [bctx.stream_open(w, h) for w, h in shapes]
@@ -239,3 +242,45 @@ def test_invalid_arguments(S):
         with pytest.raises(S.SvgfError) as e:
             ctx.stream_open(w, h)
         assert e.value.status == 4
+
+
+def test_persistent_wait_limit_reports_timeout(S, monkeypatch):
+    """The persistent form's bounded dependency wait, forced from the host: with a 0 us limit
+    every wait that finds its dependency unfinished gives up at once (the kernel still ends
+    normally), and the next call that reuses the slot returns SVGF_ERR_TIMEOUT. A context with
+    the default limit on the same inputs is unaffected. One 1280x720 frame per batch: its
+    stage i + 1 tickets directly follow its stage i tickets, so workgroups do wait (in a batch of
+    many frames a frame's previous stage has long finished when its next stage's tickets come)."""
+    dev = torch.device("cuda")
+    w, h = 1280, 720
+    d = S.synth_frame(w, h, seed=5, frame=0)
+    rad = torch.from_numpy(d["radiance"]).to(dev).half()
+    g = torch.from_numpy(d["gbuffer"]).to(dev)
+    mv = torch.from_numpy(d["motion"]).to(dev)
+    n = 1
+
+    def run(ctx, calls):
+        sids = [ctx.stream_open(w, h) for _ in range(n)]
+        outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for _ in range(n)]
+        statuses = []
+        for _ in range(calls):
+            items = [{"stream_id": s, "radiance": rad, "gbuffer": g, "motion": mv, "denoised": o}
+                     for s, o in zip(sids, outs)]
+            try:
+                ctx.denoise_batch(items)
+                statuses.append(0)
+            except S.SvgfError as e:
+                statuses.append(e.status)
+        torch.cuda.synchronize()
+        return statuses
+
+    kw = dict(max_width=w, max_height=h, max_streams=n, max_batch_frames=n, max_batch_pixels=n * w * h,
+              batch_flags=S.BATCH_PERSISTENT)
+    monkeypatch.setenv("SVGF_PERSISTENT_WAIT_US", "0")
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, **kw) as ctx:
+        st = run(ctx, 6)
+    monkeypatch.delenv("SVGF_PERSISTENT_WAIT_US")
+    print("statuses with a 0 us wait limit:", st)
+    assert st[:3] == [0, 0, 0] and 9 in st[3:]
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, **kw) as ctx:
+        assert run(ctx, 6) == [0] * 6

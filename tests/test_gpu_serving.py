@@ -31,10 +31,13 @@ def S():
 SHAPES = [(96, 96), (128, 128), (33, 17), (256, 64), (70, 300), (1, 1)]
 
 
-@pytest.mark.parametrize("storage", ["fp16", "fp32"])
+@pytest.mark.parametrize("storage", ["fp16", "fp32", "fp16_persistent"])
 def test_server_results_equal_alone(S, storage):
-    st = S.STORAGE_FP16 if storage == "fp16" else S.STORAGE_FP32
-    dt = torch.float16 if storage == "fp16" else torch.float32
+    """fp16_persistent: each batch is one persistent launch, and with two lanes two such
+    launches share the device."""
+    st = S.STORAGE_FP32 if storage == "fp32" else S.STORAGE_FP16
+    dt = torch.float32 if storage == "fp32" else torch.float16
+    flags = S.BATCH_PERSISTENT if storage.endswith("persistent") else 0
     dev = torch.device("cuda")
     frames = 4
     inputs = [seq_inputs(w, h, frames, seed0=100 + 7 * i, stream=i, pan_px=0.8, disocclusion=0.1)
@@ -44,7 +47,7 @@ def test_server_results_equal_alone(S, storage):
     maxh = max(h for _, h in SHAPES)
     outs = {}
     with S.Context(S.MODE_CANONICAL, st, max_width=maxw, max_height=maxh, max_streams=8, max_batch_frames=4,
-                   max_batch_pixels=npx) as ctx:
+                   max_batch_pixels=npx, batch_flags=flags) as ctx:
         with S.Server(ctx, lanes=2, max_batch_frames=4, max_batch_pixels=npx) as srv:
             sids = [srv.stream_open(w, h) for w, h in SHAPES]
             order = [(i, f) for f in range(frames) for i in range(len(SHAPES))]

@@ -8,6 +8,7 @@
 #include "svgf_internal.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <new>
@@ -54,6 +55,9 @@ struct BatchSlot {
     std::vector<uint32_t> ids;  // streams the pending batch touches
     std::vector<hipGraphNode_t> knodes;      // captured stage kernels, launch order
     std::vector<hipKernelNodeParams> kparams;
+    PersistCtl* ctl = nullptr;  // persistent form: ticket and completion counters (zero between launches)
+    uint32_t* err_host = nullptr;  // persistent form: host-mapped timeout flag
+    uint32_t* err_dev = nullptr;
 };
 
 struct DeviceGuard {
@@ -93,6 +97,8 @@ struct svgf_context {
     int next_slot = 0;
     size_t table_bytes = 0, starts_off = 0, frames_off = 0, arena_bytes = 0;
     int grid_blocks = 0;
+    uint32_t persist_blocks = 0;  // resident workgroups of the persistent kernel on the device
+    uint64_t max_polls = 0;       // persistent form: longest dependency wait, in polls
     hipStream_t capture_stream = nullptr;
 
     // SVGF_PROFILE_STAGES
@@ -187,6 +193,8 @@ void free_slot(BatchSlot& s)
     if (s.graph) (void)hipGraphDestroy(s.graph);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.staging) (void)hipHostFree(s.staging);
+    if (s.err_host) (void)hipHostFree(s.err_host);
+    if (s.ctl) (void)hipFree(s.ctl);
     if (s.table) (void)hipFree(s.table);
     if (s.arena) (void)hipFree(s.arena);
     s = BatchSlot{};
@@ -222,6 +230,30 @@ void destroy_context(svgf_context* c)
 
 int stage_count(const svgf_context* c) { return c->kp.iterations + (c->mode == SVGF_MODE_REFERENCE ? 1 : 2); }
 
+// One persistent launch per batch (stage profiling needs one launch per stage).
+bool persistent(const svgf_context* c)
+{
+    return (c->desc.batch_flags & SVGF_BATCH_PERSISTENT) && !(c->desc.batch_flags & SVGF_PROFILE_STAGES);
+}
+
+// Kernel launches per batch and the grid of launch i for a batch whose stages have
+// `tiles[i]` tiles: one block per tile, or for the persistent kernel at most as many
+// workgroups as the device holds at once (more would only queue behind them).
+int launches_per_batch(const svgf_context* c) { return persistent(c) ? 1 : stage_count(c); }
+
+// Tiles per ticket-counter increment of the persistent kernel: fewer same-address atomics and
+// neighbouring tiles on one workgroup (one L2), against a longer tail at each stage boundary.
+constexpr uint32_t PERSIST_CHUNK = 2;
+
+uint32_t launch_grid(const svgf_context* c, const uint32_t* tiles, int i)
+{
+    if (!persistent(c)) return tiles[i];
+    uint64_t total = 0;
+    for (int st = 0; st < stage_count(c); ++st) total += tiles[st];
+    const uint64_t chunks = (total + PERSIST_CHUNK - 1) / PERSIST_CHUNK;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, c->persist_blocks));
+}
+
 hipEvent_t* profile_events(svgf_context* c)
 {
     if (!(c->desc.batch_flags & SVGF_PROFILE_STAGES)) return nullptr;
@@ -229,7 +261,7 @@ hipEvent_t* profile_events(svgf_context* c)
     return c->ev;
 }
 
-hipError_t enqueue_batch(svgf_context* c, BatchSlot& s, const uint32_t* grids, hipStream_t stream,
+hipError_t enqueue_batch(svgf_context* c, BatchSlot& s, const uint32_t* tiles, hipStream_t stream,
                          hipEvent_t* ev = nullptr)
 {
     hipError_t e = hipMemcpyAsync(s.table, s.staging, c->table_bytes, hipMemcpyHostToDevice, stream);
@@ -237,17 +269,21 @@ hipError_t enqueue_batch(svgf_context* c, BatchSlot& s, const uint32_t* grids, h
     char* t = static_cast<char*>(s.table);
     BatchTable bt{reinterpret_cast<const uint32_t*>(t), reinterpret_cast<const uint32_t*>(t + c->starts_off),
                   reinterpret_cast<const FrameDesc*>(t + c->frames_off)};
+    if (persistent(c))
+        return launch_batch_persistent(c->mode == SVGF_MODE_CANONICAL, bt, c->kp, c->st, s.ctl, s.err_dev, c->max_polls,
+                                       launch_grid(c, tiles, 0), stream);
+    const uint32_t* grids = tiles;
     if (c->mode == SVGF_MODE_REFERENCE) return launch_reference_batch(bt, c->kp, c->st, grids, stream, ev);
     return launch_canonical_batch(bt, c->kp, c->st, grids, stream, ev);
 }
 
-// Captures table copy + stage launches of a slot once; later batches only change the grid
-// size of each kernel node (one block per tile), so the graph is never re-instantiated.
-hipError_t capture_slot(svgf_context* c, BatchSlot& s, const uint32_t* grids)
+// Captures table copy + kernel launches of a slot once; later batches only change the grid
+// size of each kernel node, so the graph is never re-instantiated.
+hipError_t capture_slot(svgf_context* c, BatchSlot& s, const uint32_t* tiles)
 {
     hipError_t e = hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    hipError_t le = enqueue_batch(c, s, grids, c->capture_stream);
+    hipError_t le = enqueue_batch(c, s, tiles, c->capture_stream);
     e = hipStreamEndCapture(c->capture_stream, &s.graph);
     if (le != hipSuccess) e = le;
     if (e == hipSuccess) e = hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0);
@@ -277,7 +313,7 @@ hipError_t capture_slot(svgf_context* c, BatchSlot& s, const uint32_t* grids)
         e = hipGraphNodeGetDependentNodes(node, &next, &nd);
         node = next;
     }
-    if (e == hipSuccess && (int)s.knodes.size() != stage_count(c)) e = hipErrorUnknown;
+    if (e == hipSuccess && (int)s.knodes.size() != launches_per_batch(c)) e = hipErrorUnknown;
     if (e != hipSuccess) {
         if (s.exec) (void)hipGraphExecDestroy(s.exec);
         if (s.graph) (void)hipGraphDestroy(s.graph);
@@ -373,7 +409,7 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
     if ((uint64_t)d->max_width * d->max_height * 16 >= (1ull << 31)) return SVGF_ERR_CAPACITY;
     if (d->max_batch_frames > MAX_BATCH_FRAMES) return SVGF_ERR_INVALID_ARGUMENT;
     if (d->max_batch_frames && d->max_batch_pixels < 1) return SVGF_ERR_INVALID_ARGUMENT;
-    if (d->batch_flags & ~(SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES)) return SVGF_ERR_INVALID_ARGUMENT;
+    if (d->batch_flags & ~(SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES | SVGF_BATCH_PERSISTENT)) return SVGF_ERR_INVALID_ARGUMENT;
     svgf_status ps = validate_params(d->params, d->mode);
     if (ps != SVGF_OK) return ps;
 
@@ -394,6 +430,17 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
     c->w4 = c->mode == SVGF_MODE_CANONICAL ? 16 : c->s4;
     c->kp = to_kparams(d->params);
     c->grid_blocks = std::max(1, prop.multiProcessorCount) * 8;
+    {
+        const int per_cu = persistent_blocks_per_cu(c->mode == SVGF_MODE_CANONICAL, c->st, c->kp.compensated != 0);
+        c->persist_blocks = (uint32_t)std::max(1, per_cu * std::max(1, prop.multiProcessorCount));
+        // dependency-wait bound of the persistent kernel, in polls of about 1.3 us (a 1024-cycle
+        // sleep plus one load): 250 ms
+        uint64_t us = 250000;
+        // diagnostic override in microseconds (0 makes every wait that finds its dependency
+        // unfinished give up at once: the error path without a stuck kernel)
+        if (const char* env = std::getenv("SVGF_PERSISTENT_WAIT_US")) us = std::strtoull(env, nullptr, 10);
+        c->max_polls = us * 10 / 13;
+    }
 
     uint64_t sb = frame_scratch_bytes(c->w4,(uint64_t)d->max_width * d->max_height);
     if (hipMalloc(&c->scratch, sb) != hipSuccess) {
@@ -424,6 +471,20 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
                 return SVGF_ERR_OUT_OF_MEMORY;
             }
             std::memset(s.staging, 0, c->table_bytes);
+            // persistent form: counters start (and are left by every launch) at zero; the
+            // timeout flag lives in host-mapped memory so the host reads it without a copy
+            if (hipMalloc(reinterpret_cast<void**>(&s.ctl), sizeof(PersistCtl)) != hipSuccess ||
+                hipMemset(s.ctl, 0, sizeof(PersistCtl)) != hipSuccess ||
+                hipHostMalloc(reinterpret_cast<void**>(&s.err_host), sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+                hipHostGetDevicePointer(reinterpret_cast<void**>(&s.err_dev), s.err_host, 0) != hipSuccess) {
+                destroy_context(c);
+                return SVGF_ERR_OUT_OF_MEMORY;
+            }
+            *s.err_host = 0;
+        }
+        if (hipDeviceSynchronize() != hipSuccess) {
+            destroy_context(c);
+            return SVGF_ERR_HIP;
         }
         if (hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking) != hipSuccess) {
             destroy_context(c);
@@ -655,6 +716,11 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
         if (hipEventSynchronize(slot.done) != hipSuccess) return SVGF_ERR_HIP;
         slot.pending = false;
     }
+    if (*slot.err_host) {
+        // a dependency wait of this slot's last persistent launch gave up
+        *slot.err_host = 0;
+        return SVGF_ERR_TIMEOUT;
+    }
     char* host = static_cast<char*>(slot.staging);
     uint32_t* hdr = reinterpret_cast<uint32_t*>(host);
     uint32_t* starts = reinterpret_cast<uint32_t*>(host + c->starts_off);
@@ -683,34 +749,29 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
         arena += frame_scratch_bytes(c->w4,(uint64_t)s.w * s.h);
         frames[i] = f;
     }
-    // per stage: first tile of each frame (stage strides differ, so do the tile counts)
-    const int canonical = c->mode == SVGF_MODE_CANONICAL;
+    // per stage: first tile of each frame (stage strides differ, so do the tile counts), and
+    // the first ticket of each stage for the persistent form
     const int nstages = stage_count(c);
-    uint32_t grids[MAX_STAGES];
-    for (int st = 0; st < nstages; ++st) {
-        uint32_t* row = starts + (size_t)st * STARTS_ROW;
-        uint32_t tile = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            row[i] = tile;
-            tile += tiles_at((uint32_t)frames[i].w, (uint32_t)frames[i].h, stage_log2s(canonical, st));
-        }
-        row[n] = tile;
-        grids[st] = tile;
-    }
-    hdr[0] = n;
+    const uint64_t total = build_schedule(
+        n, [frames](uint32_t i, uint32_t& w, uint32_t& h) { w = (uint32_t)frames[i].w; h = (uint32_t)frames[i].h; },
+        c->mode == SVGF_MODE_CANONICAL, nstages, hdr, starts);
+    if (total >= (1ull << 32) - (1ull << 24)) return SVGF_ERR_CAPACITY;  // tickets are 32-bit
+    hdr[HDR_CHUNK] = PERSIST_CHUNK;
+    uint32_t tiles[MAX_STAGES];
+    for (int st = 0; st < nstages; ++st) tiles[st] = starts[(size_t)st * STARTS_ROW + n];
 
     hipStream_t stream = static_cast<hipStream_t>(hip_stream);
     hipError_t e = hipSuccess;
     if (c->desc.batch_flags & (SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES)) {
-        e = enqueue_batch(c, slot, grids, stream, profile_events(c));
+        e = enqueue_batch(c, slot, tiles, stream, profile_events(c));
     } else {
         if (!slot.exec) {
-            if (capture_slot(c, slot, grids) != hipSuccess) return SVGF_ERR_HIP;
+            if (capture_slot(c, slot, tiles) != hipSuccess) return SVGF_ERR_HIP;
         } else {
-            for (int st = 0; st < nstages && e == hipSuccess; ++st) {
-                hipKernelNodeParams kp = slot.kparams[st];
-                kp.gridDim = dim3(grids[st], 1, 1);
-                e = hipGraphExecKernelNodeSetParams(slot.exec, slot.knodes[st], &kp);
+            for (int i = 0; i < launches_per_batch(c) && e == hipSuccess; ++i) {
+                hipKernelNodeParams kp = slot.kparams[i];
+                kp.gridDim = dim3(launch_grid(c, tiles, i), 1, 1);
+                e = hipGraphExecKernelNodeSetParams(slot.exec, slot.knodes[i], &kp);
             }
             if (e != hipSuccess) return SVGF_ERR_HIP;
         }

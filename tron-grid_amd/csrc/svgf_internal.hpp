@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "svgf_sched.hpp"
+
 namespace svgf {
 
 // A pitched 2D plane as the kernels see it. Kernels address it through a raw buffer resource
@@ -47,43 +49,14 @@ struct KParams {
     int32_t compensated;
 };
 
-// Work decomposition. Every stage runs on 16x16 tiles of a stride-s lattice: for a-trous
-// iteration k (s = 2^k) the pixels of one residue class (x mod s, y mod s) form a lattice
-// whose 5x5 taps are the lattice neighbours, so a tile plus a 2-point halo (20x20 entries)
-// holds every tap in LDS whatever the stride. Pointwise stages use s = 1 (plain pixel tiles).
-// A frame at stride s has s^2 * ceil(ceil(w/s)/16) * ceil(ceil(h/s)/16) tiles, ordered tile-
-// major with the residue fastest (the s^2 blocks of one 16s x 16s pixel region are adjacent).
-constexpr int LT = 16;
-constexpr int LHALO = 2;
-constexpr int LREG = LT + 2 * LHALO;  // 20
-constexpr int BLOCK = LT * LT;        // 256 threads
-
-__host__ __device__ inline uint32_t tiles_at(uint32_t w, uint32_t h, uint32_t log2s)
-{
-    uint32_t s = 1u << log2s;
-    uint32_t lx = (w + s - 1) >> log2s, ly = (h + s - 1) >> log2s;
-    return (s * s) * ((lx + LT - 1) / LT) * ((ly + LT - 1) / LT);
-}
-
-// Stage list of a mode: REFERENCE = init + N a-trous; CANONICAL = temporal, variance,
-// N a-trous. log2 stride of stage i:
-__host__ __device__ inline uint32_t stage_log2s(int canonical, int stage)
-{
-    int first_atrous = canonical ? 2 : 1;
-    return stage < first_atrous ? 0u : (uint32_t)(stage - first_atrous);
-}
-constexpr int MAX_STAGES = 12;
-constexpr uint32_t MAX_BATCH_FRAMES = 1024;
-
-// Batched form: the device-side table of a batch. hdr[0] = frames; for stage i, frame f owns
-// tiles [starts[i * (MAX_BATCH_FRAMES + 1) + f], ... + f + 1); a block maps its tile index to
-// a frame by binary search over that row (uniform across the block: scalar unit).
+// Batched form: the device-side table of a batch (header and starts rows as in
+// svgf_sched.hpp); a block maps its tile index to a frame by binary search over the stage's
+// row (uniform across the block: scalar unit).
 struct BatchTable {
     const uint32_t* hdr;
     const uint32_t* starts;
     const FrameDesc* frames;
 };
-constexpr uint32_t STARTS_ROW = MAX_BATCH_FRAMES + 1;
 
 enum class Storage : int { F32 = 0, F16 = 1 };
 
@@ -100,6 +73,15 @@ hipError_t launch_reference_batch(const BatchTable& t, const KParams& p, Storage
                                   hipStream_t stream, hipEvent_t* ev = nullptr);
 hipError_t launch_canonical_batch(const BatchTable& t, const KParams& p, Storage st, const uint32_t* grid_blocks,
                                   hipStream_t stream, hipEvent_t* ev = nullptr);
+
+// Persistent batched form: one launch of `blocks` workgroups runs every stage of the batch,
+// taking tickets from *ctl (protocol in svgf_sched.hpp). *ctl must be zero before the first
+// launch and is zero again after each one. *err (host-mapped) becomes 1 when a dependency wait
+// gave up after max_polls polls.
+hipError_t launch_batch_persistent(int canonical, const BatchTable& t, const KParams& p, Storage st, PersistCtl* ctl,
+                                   uint32_t* err, uint64_t max_polls, uint32_t blocks, hipStream_t stream);
+// Resident workgroups per CU of the persistent kernel (0 if the query failed).
+int persistent_blocks_per_cu(int canonical, Storage st, bool compensated);
 
 }  // namespace svgf
 

@@ -124,6 +124,22 @@ __device__ __forceinline__ float2 ldg2(const Buf& b, int x, int y) { return ld_f
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Completes this wave's outstanding LDS operations (s_waitcnt lgkmcnt(0)). A value one thread
+// writes to LDS for the other waves must have landed before that thread's wave reaches the
+// barrier that publishes it; the compiler does not always place this wait before a barrier
+// reached through a loop back edge, and a wave that passes the barrier early reads the old value.
+__device__ __forceinline__ void lds_writes_done() { __builtin_amdgcn_s_waitcnt(0xc07f); }
+
+// The thread index as a value the optimiser cannot see through. Stage functions derive their
+// LDS addresses from it; in the persistent kernel's ticket loop those addresses would
+// otherwise be hoisted out of the loop one per tap, each holding a VGPR for the whole kernel.
+__device__ __forceinline__ int lane_tid()
+{
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // computeLuminance (svgf_denoise.slang:109-112) rounded as the shader's dot: three products,
 // two sums, left to right, no fused multiply-add.
 __device__ __forceinline__ float luma_ref(float r, float g, float b)
@@ -210,10 +226,11 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t n)
 
 // Pass 0 (svgf_denoise.slang:176-191): moments EMA, variance, sqrt into wavelet_a alpha.
 template <class S>
-__device__ void ref_init_tile(const FrameDesc& f, const KParams& p, const Tile& T)
+__device__ __forceinline__ void ref_init_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
 #pragma clang fp contract(off)
-    const int x = T.i0 + (int)(threadIdx.x & (LT - 1)), y = T.j0 + (int)(threadIdx.x >> 4);
+    const int tid = lane_tid();
+    const int x = T.i0 + (tid & (LT - 1)), y = T.j0 + (tid >> 4);
     if (x >= f.w || y >= f.h) return;
     const Buf rad = bind(f.rad, f.h), mc = bind(f.mom_curr, f.h), wa = bind(f.wa, f.h);
     float4 L = S::ld4(rad, x, y);
@@ -234,13 +251,13 @@ __device__ void ref_init_tile(const FrameDesc& f, const KParams& p, const Tile& 
 // Passes 1..N (svgf_denoise.slang:209-274): 5x5 B3-spline taps at `stride`, clamp-to-edge,
 // weight K[dx] K[dy] exp(-|l_q - l_c| / max(phi * sigma_c, eps)), dy outer / dx inner.
 template <class S>
-__device__ void ref_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int pass)
+__device__ __forceinline__ void ref_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int pass)
 {
     __shared__ float4 sA[LREG * LREG];  // (r, g, b, luminance) of the tile + halo
     const bool source_is_a = (pass & 1) == 1;
     const Buf src = bind(source_is_a ? f.wa : f.wb, f.h);
     const int max_x = f.w - 1, max_y = f.h - 1;
-    const int tid = (int)threadIdx.x;
+    const int tid = lane_tid();
     for (int e = tid; e < LREG * LREG; e += BLOCK) {
         int li = e % LREG, lj = e / LREG;
         int qx = clampi(T.px(T.i0 - LHALO + li), 0, max_x), qy = clampi(T.py(T.j0 - LHALO + lj), 0, max_y);
@@ -304,9 +321,10 @@ __device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(
 // also keeps G for the next frame. Standard deviations rather than variances are stored so
 // that heavy-tailed radiance stays inside the FP16 range.
 template <class S, bool COMP>
-__device__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Tile& T)
+__device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
-    const int x = T.i0 + (int)(threadIdx.x & (LT - 1)), y = T.j0 + (int)(threadIdx.x >> 4);
+    const int tid = lane_tid();
+    const int x = T.i0 + (tid & (LT - 1)), y = T.j0 + (tid >> 4);
     if (x >= f.w || y >= f.h) return;
     const Buf rad = bind(f.rad, f.h), gb = bind(f.g, f.h);
     float4 R = S::ld4(rad, x, y);
@@ -447,14 +465,14 @@ __device__ __forceinline__ float edge_log2(const KParams& p, float4 Gc, float4 G
 // temporal variance the temporal stage wrote. (Only the alpha bytes are written; neighbours
 // read only colour bytes, so the in-place update has no read/write overlap.)
 template <class S>
-__device__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Tile& T)
+__device__ __forceinline__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
     // Young pixels are scattered (disocclusions), so nearly every wave holds a few of them.
     // Compact them first: the tile's young pixels go to the leading lanes of the block, and
     // only ceil(count / 64) waves run the 3x3 window.
     __shared__ int sList[BLOCK];
     __shared__ int sCount;
-    const int tid = (int)threadIdx.x;
+    const int tid = lane_tid();
     {
         const int x = T.i0 + (tid & (LT - 1)), y = T.j0 + (tid >> 4);
         bool young = false;
@@ -463,8 +481,10 @@ __device__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Ti
             young = ld_u8(nl, y * nl.pitch + x) < 4u;
         }
         if (tid == 0) sCount = 0;
+        lds_writes_done();
         __syncthreads();
         if (young) sList[atomicAdd(&sCount, 1)] = tid;
+        lds_writes_done();
         __syncthreads();
     }
     if (tid >= sCount) return;
@@ -509,7 +529,7 @@ __device__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Ti
 // edge-stops, variance filtered by squared weights. Iteration 0 also writes the colour
 // history, the last one the output.
 template <class S>
-__device__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
+__device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
 {
     __shared__ float4 sA[LREG * LREG];  // (r, g, b, luminance)
     __shared__ float4 sB[LREG * LREG];  // (normal, depth)
@@ -517,7 +537,7 @@ __device__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile
     const int s = 1 << k;
     const Buf src = bind((k & 1) ? f.wb : f.wa, f.h), gb = bind(f.g, f.h);
     const int max_x = f.w - 1, max_y = f.h - 1;
-    const int tid = (int)threadIdx.x;
+    const int tid = lane_tid();
     for (int e = tid; e < LREG * LREG; e += BLOCK) {
         int li = e % LREG, lj = e / LREG;
         int qx = clampi(T.px(T.i0 - LHALO + li), 0, max_x), qy = clampi(T.py(T.j0 - LHALO + lj), 0, max_y);
@@ -607,6 +627,17 @@ __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, 
     else can_atrous_tile<S>(f, p, T, arg);
 }
 
+// Frame descriptor of a batch entry into scalar registers (the index is uniform across the block).
+__device__ __forceinline__ FrameDesc load_frame(const FrameDesc* frames, uint32_t i)
+{
+    FrameDesc f;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(frames + i);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&f);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(FrameDesc) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+    return f;
+}
+
 // arg: REFERENCE a-trous pass (1..N) or CANONICAL a-trous iteration (0..N-1); log2s: stride.
 template <class S, int STAGE, bool COMP>
 __global__ __launch_bounds__(BLOCK) void k_frame(FrameDesc f, KParams p, int arg, uint32_t log2s)
@@ -620,29 +651,151 @@ __global__ __launch_bounds__(BLOCK) void k_frame(FrameDesc f, KParams p, int arg
 template <class S, int STAGE, bool COMP>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchTable bt, KParams p, int arg, uint32_t log2s, int stage)
 {
-    const uint32_t nf = bt.hdr[0];
+    const uint32_t nf = bt.hdr[HDR_FRAMES];
     const uint32_t* starts = bt.starts + (uint32_t)stage * STARTS_ROW;
     const uint32_t t = xcd_logical(blockIdx.x, gridDim.x);
     if (t >= starts[nf]) return;
-    // largest frame index i with starts[i] <= t
-    uint32_t lo = 0, hi = nf - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (starts[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    // frame descriptor into scalar registers (the index is uniform across the block)
-    FrameDesc f;
-    {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(bt.frames + lo);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&f);
-#pragma unroll
-        for (int i = 0; i < (int)(sizeof(FrameDesc) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
-    }
-    Tile T = decode_tile(f.w, log2s, t - starts[lo]);
+    const uint32_t fi = frame_of(starts, nf, t);
+    const FrameDesc f = load_frame(bt.frames, fi);
+    Tile T = decode_tile(f.w, log2s, t - starts[fi]);
     run_stage<S, STAGE, COMP>(f, p, T, arg);
 }
 
+// One tile of a persistent launch. Each stage is its own (non-inlined) function that loads the
+// frame descriptor itself: inlined into one kernel, the stages would keep every descriptor field
+// live across all of them and the register count would rise to that of all stages together.
+template <class S, int STAGE, bool COMP>
+__device__ __forceinline__ void persist_tile(const FrameDesc* frames, uint32_t fi, const KParams& p, uint32_t log2s,
+                                          uint32_t tile, int arg)
+{
+    const FrameDesc f = load_frame(frames, fi);
+    run_stage<S, STAGE, COMP>(f, p, decode_tile(f.w, log2s, tile), arg);
+}
+
+// Persistent form: every stage of a batch in one launch. Each workgroup takes chunks of
+// consecutive tickets from ctl->ticket until none is left (order: svgf_sched.hpp; a chunk's
+// tiles are neighbours, which then share one XCD's L2). Stage i > 0 may start once every
+// tile of stage i - 1 (all frames) is done; a workgroup waits for that once per stage, then
+// acquires (every wave invalidates its caches' stale lines of the previous stage's outputs).
+// Finished tiles are published lazily: a workgroup counts them and publishes the count (every
+// wave makes its stores visible at device scope, then one atomic add) when its next ticket
+// belongs to another stage or none is left - always before it waits. A workgroup that waits
+// holds no unpublished tile and only waits for tickets smaller than its own, so the holder of
+// the smallest unfinished ticket can always run: no deadlock, whether or not all workgroups
+// are resident at once. Per-stage (not per-frame) dependencies also mean that when stage i
+// starts no read of a stage < i - 1 is still in flight, so nothing refills a cache with stale
+// lines after the acquire. A wait of more than max_polls polls (about 1.3 us each) gives up,
+// sets *err (host memory) and ctl->abort, and skips its tile; once ctl->abort is set every
+// wait gives up at once, so the kernel always ends soon after. The last workgroup to find
+// the tickets exhausted clears the counters, leaving ctl zeroed for the next launch.
+template <class S, bool CANON, bool COMP>
+__global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, PersistCtl* ctl, uint32_t* err,
+                                                   uint64_t max_polls)
+{
+    // Thread 0 writes the next chunk index into the slot the other waves are not reading; the
+    // barrier at the top of the loop orders both uses of a slot.
+    __shared__ uint32_t sChunk[2];
+    __shared__ int sFlag;
+    const int tid = (int)threadIdx.x;
+    const uint32_t* hdr = bt.hdr;
+    const uint32_t nstages = hdr[HDR_STAGES];
+    const uint32_t total = hdr[HDR_BASE + nstages];
+    const uint32_t chunk = hdr[HDR_CHUNK];
+    // Thread 0 takes a chunk in one block at the end of the loop body (the first one before
+    // the loop); a `tid == 0` block on each side of the back edge would be threaded into one
+    // and give thread 0 a loop path of its own, so the other lanes would reach the next barrier
+    // without it.
+    if (tid == 0) {
+        sChunk[0] = __hip_atomic_fetch_add(&ctl->ticket[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lds_writes_done();
+    }
+    uint32_t slot = 0, t = 0, end = 0;        // tickets [t, end) of the current chunk
+    uint32_t pending = 0, pending_stage = 0;  // finished, unpublished tiles (all of one stage)
+    uint32_t ready_stage = 0;                 // highest stage whose inputs this workgroup has acquired
+    for (;;) {
+        __syncthreads();
+        if (t == end) {
+            const uint32_t c = __builtin_amdgcn_readfirstlane(sChunk[slot]);
+            t = c < total / chunk + 1 ? c * chunk : total;  // (no overflow of c * chunk)
+            end = min(t + chunk, total);
+        }
+        const bool more = t < total;
+        Ticket k{0, 0, 0};
+        if (more) k = decode_ticket(hdr, bt.starts, t);
+        if (pending != 0 && (!more || k.stage != pending_stage)) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (tid == 0)
+                __hip_atomic_fetch_add(&ctl->done[pending_stage * CTL_LINE], pending, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            pending = 0;
+        }
+        if (!more) break;
+        bool run = true;
+        if (k.stage > ready_stage) {
+            if (tid == 0) {
+                const uint32_t need = hdr[HDR_BASE + k.stage] - hdr[HDR_BASE + k.stage - 1];
+                uint32_t* c = &ctl->done[(k.stage - 1) * CTL_LINE];
+                int ok = 1;
+                // bounded by a number of polls, not by a clock: a wave that is swapped out
+                // (queue preemption) does not poll, so the bound cannot expire while it is away
+                for (uint64_t polls = 0; __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need; ++polls) {
+                    if (polls >= max_polls ||
+                        ((polls & 7) == 7 && __hip_atomic_load(&ctl->abort[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(16);
+                }
+                if (!ok) {
+                    __hip_atomic_store(&ctl->abort[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // a plain store (no read-modify-write over the host link): only 1 is written
+                    __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+                sFlag = ok;
+                lds_writes_done();
+            }
+            __syncthreads();
+            // every wave: later loads must not hit lines cached before the producers' stores
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            run = __builtin_amdgcn_readfirstlane(sFlag) != 0;
+            if (run) ready_stage = k.stage;
+        }
+        if (run) {
+            if constexpr (CANON) {
+                if (k.stage == 0) persist_tile<S, ST_CAN_TEMPORAL, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
+                else if (k.stage == 1) persist_tile<S, ST_CAN_VARIANCE, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
+                else persist_tile<S, ST_CAN_ATROUS, COMP>(bt.frames, k.frame, p, k.stage - 2, k.tile, (int)k.stage - 2);
+            } else {
+                if (k.stage == 0) persist_tile<S, ST_REF_INIT, false>(bt.frames, k.frame, p, 0, k.tile, 0);
+                else persist_tile<S, ST_REF_ATROUS, false>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage);
+            }
+        }
+        ++pending;
+        pending_stage = k.stage;
+        if (++t == end) {
+            slot ^= 1u;
+            if (tid == 0) {
+                sChunk[slot] = __hip_atomic_fetch_add(&ctl->ticket[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                lds_writes_done();
+            }
+        }
+    }
+    if (tid == 0) {
+        sFlag = __hip_atomic_fetch_add(&ctl->exited[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+        lds_writes_done();
+    }
+    __syncthreads();
+    if (sFlag) {
+        // every other workgroup has exited: nothing reads the counters any more
+        if (tid < (int)nstages) __hip_atomic_store(&ctl->done[tid * CTL_LINE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+            __hip_atomic_store(&ctl->ticket[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->exited[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->abort[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
 template <class S, int STAGE, bool COMP>
 void launch_frame(const FrameDesc& f, const KParams& p, int arg, uint32_t log2s, hipStream_t stream)
 {
@@ -714,6 +867,22 @@ void canonical_batch(const BatchTable& bt, const KParams& p, const uint32_t* g, 
     }
 }
 
+template <class S, bool CANON, bool COMP>
+hipError_t persist_launch(const BatchTable& bt, const KParams& p, PersistCtl* ctl, uint32_t* err, uint64_t max_polls,
+                          uint32_t blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_persist<S, CANON, COMP>), dim3(blocks), dim3(BLOCK), 0, s, bt, p, ctl, err, max_polls);
+    return hipGetLastError();
+}
+
+template <class S, bool CANON, bool COMP>
+int persist_occupancy()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persist<S, CANON, COMP>, BLOCK, 0) != hipSuccess) return 0;
+    return n;
+}
+
 }  // namespace
 
 hipError_t launch_reference_frame(const FrameDesc& f, const KParams& p, Storage st, hipStream_t stream, hipEvent_t* ev)
@@ -754,6 +923,29 @@ hipError_t launch_canonical_batch(const BatchTable& bt, const KParams& p, Storag
         else canonical_batch<F16S, false>(bt, p, grid_blocks, stream, ev);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_batch_persistent(int canonical, const BatchTable& bt, const KParams& p, Storage st, PersistCtl* ctl,
+                                   uint32_t* err, uint64_t max_polls, uint32_t blocks, hipStream_t stream)
+{
+    if (!canonical) {
+        if (st == Storage::F32) return persist_launch<F32S, false, false>(bt, p, ctl, err, max_polls, blocks, stream);
+        return persist_launch<F16S, false, false>(bt, p, ctl, err, max_polls, blocks, stream);
+    }
+    if (st == Storage::F32) {
+        if (p.compensated) return persist_launch<F32S, true, true>(bt, p, ctl, err, max_polls, blocks, stream);
+        return persist_launch<F32S, true, false>(bt, p, ctl, err, max_polls, blocks, stream);
+    }
+    if (p.compensated) return persist_launch<F16S, true, true>(bt, p, ctl, err, max_polls, blocks, stream);
+    return persist_launch<F16S, true, false>(bt, p, ctl, err, max_polls, blocks, stream);
+}
+
+int persistent_blocks_per_cu(int canonical, Storage st, bool compensated)
+{
+    if (!canonical) return st == Storage::F32 ? persist_occupancy<F32S, false, false>() : persist_occupancy<F16S, false, false>();
+    if (st == Storage::F32)
+        return compensated ? persist_occupancy<F32S, true, true>() : persist_occupancy<F32S, true, false>();
+    return compensated ? persist_occupancy<F16S, true, true>() : persist_occupancy<F16S, true, false>();
 }
 
 }  // namespace svgf

@@ -1,0 +1,138 @@
+// svgf_sched.hpp — tile decomposition and the work order of the batched kernels.
+//
+// Plain integer code with no HIP dependency: the kernels use it on the device, and a host
+// simulation of the persistent kernel's ticket protocol (tests/sched_sim.cpp) compiles the
+// same functions with g++.
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#define SVGF_HD __host__ __device__
+#else
+#define SVGF_HD
+#endif
+
+namespace svgf {
+
+// Every stage runs on 16x16 tiles of a stride-s lattice: for a-trous iteration k (s = 2^k) the
+// pixels of one residue class (x mod s, y mod s) form a lattice whose 5x5 taps are the lattice
+// neighbours, so a tile plus a 2-point halo (20x20 entries) holds every tap in LDS whatever the
+// stride. Pointwise stages use s = 1 (plain pixel tiles). A frame at stride s has
+// s^2 * ceil(ceil(w/s)/16) * ceil(ceil(h/s)/16) tiles, ordered tile-major with the residue
+// fastest (the s^2 blocks of one 16s x 16s pixel region are adjacent).
+constexpr int LT = 16;
+constexpr int LHALO = 2;
+constexpr int LREG = LT + 2 * LHALO;  // 20
+constexpr int BLOCK = LT * LT;        // 256 threads
+
+SVGF_HD inline uint32_t tiles_at(uint32_t w, uint32_t h, uint32_t log2s)
+{
+    uint32_t s = 1u << log2s;
+    uint32_t lx = (w + s - 1) >> log2s, ly = (h + s - 1) >> log2s;
+    return (s * s) * ((lx + LT - 1) / LT) * ((ly + LT - 1) / LT);
+}
+
+// Stage list of a mode: REFERENCE = init + N a-trous; CANONICAL = temporal, variance,
+// N a-trous. log2 stride of stage i:
+SVGF_HD inline uint32_t stage_log2s(int canonical, int stage)
+{
+    int first_atrous = canonical ? 2 : 1;
+    return stage < first_atrous ? 0u : (uint32_t)(stage - first_atrous);
+}
+constexpr int MAX_STAGES = 12;
+constexpr uint32_t MAX_BATCH_FRAMES = 1024;
+constexpr uint32_t STARTS_ROW = MAX_BATCH_FRAMES + 1;
+
+// Header words of a batch table (16 words):
+//   hdr[HDR_FRAMES]          frames in the batch (>= 1)
+//   hdr[HDR_STAGES]          stages per frame (<= MAX_STAGES)
+//   hdr[HDR_BASE + i]        first ticket of stage i (i <= stages; the last entry is the total)
+// starts[i * STARTS_ROW + f] is the first tile of frame f within stage i (prefix sums of the
+// frames' tile counts; starts[i * STARTS_ROW + frames] = tiles of stage i).
+//   hdr[HDR_CHUNK]           tiles per ticket-counter increment of the persistent kernel
+constexpr int HDR_FRAMES = 0;
+constexpr int HDR_STAGES = 1;
+constexpr int HDR_BASE = 2;
+constexpr int HDR_CHUNK = 15;
+constexpr int HDR_WORDS = 16;
+static_assert(HDR_BASE + MAX_STAGES + 1 <= HDR_CHUNK, "batch header too small");
+
+// Largest frame index f with row[f] <= t (row is non-decreasing, row[0] = 0 <= t).
+SVGF_HD inline uint32_t frame_of(const uint32_t* row, uint32_t frames, uint32_t t)
+{
+    uint32_t lo = 0, hi = frames - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (row[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The persistent batched kernel hands out tickets 0 .. total-1 from one atomic counter, in
+// chunks of hdr[HDR_CHUNK] consecutive tickets per increment. Ticket order is stage-major (every tile of stage i before any tile of stage i + 1), frame-major
+// inside a stage. A tile of stage i > 0 may start once every tile of stage i - 1 has finished;
+// those tiles all hold smaller tickets, so this order is a topological order of the
+// dependencies.
+struct Ticket {
+    uint32_t stage, frame, tile;
+};
+
+SVGF_HD inline Ticket decode_ticket(const uint32_t* hdr, const uint32_t* starts, uint32_t t)
+{
+    const uint32_t ns = hdr[HDR_STAGES];
+    uint32_t st = 0;
+    while (st + 1 < ns && hdr[HDR_BASE + st + 1] <= t) ++st;
+    const uint32_t local = t - hdr[HDR_BASE + st];
+    const uint32_t* row = starts + (uint64_t)st * STARTS_ROW;
+    const uint32_t f = frame_of(row, hdr[HDR_FRAMES], local);
+    return Ticket{st, f, local - row[f]};
+}
+
+// Fills the header and the starts rows of a batch of n frames (1 <= n <= MAX_BATCH_FRAMES)
+// for a mode with `stages` stages; size(i, w, h) gives frame i's size. Returns the total
+// number of tickets (tiles over all stages); the header's ticket fields are valid only when
+// that total is below 2^32.
+template <class SizeOf>
+inline uint64_t build_schedule(uint32_t n, SizeOf size, int canonical, int stages, uint32_t* hdr, uint32_t* starts)
+{
+    uint64_t ticket = 0;
+    for (int st = 0; st < stages; ++st) {
+        uint32_t* row = starts + (uint64_t)st * STARTS_ROW;
+        uint32_t tile = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t w = 0, h = 0;
+            size(i, w, h);
+            row[i] = tile;
+            tile += tiles_at(w, h, stage_log2s(canonical, st));
+        }
+        row[n] = tile;
+        hdr[HDR_BASE + st] = (uint32_t)ticket;
+        ticket += tile;
+    }
+    hdr[HDR_BASE + stages] = (uint32_t)ticket;
+    hdr[HDR_FRAMES] = n;
+    hdr[HDR_STAGES] = (uint32_t)stages;
+    return ticket;
+}
+
+// Tiles of frame f in stage st.
+SVGF_HD inline uint32_t frame_tiles(const uint32_t* starts, uint32_t st, uint32_t f)
+{
+    const uint32_t* row = starts + (uint64_t)st * STARTS_ROW;
+    return row[f + 1] - row[f];
+}
+
+// Counters of the persistent kernel, each on a 256-byte line of its own (workgroups poll the
+// stage counters while others add to the ticket counter). The kernel zeroes what it used
+// before it ends.
+constexpr int CTL_LINE = 64;  // words
+struct PersistCtl {
+    uint32_t ticket[CTL_LINE];                // [0]: next chunk of tickets
+    uint32_t exited[CTL_LINE];                // [0]: workgroups that found no ticket left
+    uint32_t abort[CTL_LINE];                 // [0]: set by the first wait that gives up
+    uint32_t done[MAX_STAGES * CTL_LINE];     // [i * CTL_LINE]: published tiles of stage i
+};
+
+}  // namespace svgf

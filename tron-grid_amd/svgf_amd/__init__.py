@@ -31,6 +31,7 @@ FMT_RGBA32F, FMT_RGBA16F, FMT_RG32F, FMT_RG16F = 1, 2, 3, 4
 FLAG_COMPENSATED = 0x1
 BATCH_NO_GRAPH = 0x1
 PROFILE_STAGES = 0x2
+BATCH_PERSISTENT = 0x4
 
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "unsupported", 3: "out of device memory", 4: "over HBM budget",
