@@ -125,6 +125,127 @@ def precision_report(frames: int, threads: int, scale: int) -> dict:
     return out
 
 
+AGENT_SIZES = [(8, 8), (32, 32), (64, 64), (128, 128), (256, 256)]  # docs/AI_INTERFACE.md:354-360
+WIRE_BYTES_PX = 8 + 4 + 3 + 4  # fp16 colour + depth (.w of the G-buffer) in; RGB888 + float depth out
+
+
+def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
+    """One agent tick: `agents` resident streams at brain-requested eye sizes are denoised in one
+    batch (FP16 canonical), encoded to the wire (RGB888 + depth) in one launch, and the packed
+    wire buffer is pushed into the 3-slot pinned readback ring."""
+    import numpy as np
+    import torch
+
+    import svgf_amd as S
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    rng = np.random.default_rng(2)
+    sizes = [AGENT_SIZES[i] for i in rng.integers(0, len(AGENT_SIZES), agents)]
+    npx = sum(w * h for w, h in sizes)
+    rgb_off, dep_off, off = [], [], 0
+    for w, h in sizes:
+        rgb_off.append(off)
+        off += (3 * w * h + 15) // 16 * 16
+    for w, h in sizes:
+        dep_off.append(off)
+        off += 4 * w * h
+    packed = torch.empty(off, dtype=torch.uint8, device=dev)
+    ins, gbs = [], []
+    for i, (w, h) in enumerate(sizes):
+        fr = []
+        for f in range(2):
+            d = S.synth_frame(w, h, seed=77 * i + f, stream=i, frame=f, pan_px=0.5, disocclusion=0.05)
+            fr.append({"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                       "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                       "motion": torch.from_numpy(d["motion"]).to(dev)})
+        ins.append(fr)
+    outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes]
+    ctx = S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=256, max_height=256, max_streams=agents,
+                    max_batch_frames=agents, max_batch_pixels=npx)
+    wire = S.Wire(max_frames=agents)
+    rb = S.Readback(off, slots=3)
+    sids = [ctx.stream_open(w, h) for w, h in sizes]
+    # request and wire descriptors built once (the timed loop only enqueues)
+    reqs, wfs = [], []
+    for f in range(2):
+        arr = (S.Request * agents)()
+        warr = (S.WireFrame * agents)()
+        for i, (w, h) in enumerate(sizes):
+            r = arr[i]
+            r.stream_id = sids[i]
+            r.radiance = S.plane_of(ins[i][f]["radiance"], S.FMT_RGBA16F)
+            r.gbuffer = S.plane_of(ins[i][f]["gbuffer"], S.FMT_RGBA32F)
+            r.motion = S.plane_of(ins[i][f]["motion"], S.FMT_RG32F)
+            r.denoised = S.plane_of(outs[i], S.FMT_RGBA16F)
+            q = warr[i]
+            q.width, q.height = w, h
+            q.colour = S.plane_of(outs[i], S.FMT_RGBA16F)
+            q.gbuffer = S.plane_of(ins[i][f]["gbuffer"], S.FMT_RGBA32F)
+            q.rgb = packed.data_ptr() + rgb_off[i]
+            q.depth = packed.data_ptr() + dep_off[i]
+        reqs.append(arr)
+        wfs.append(warr)
+    L = S.lib()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    wp = S.wire_default_params()
+    seq = ctypes.c_uint64()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    t_den = t_enc = t_rb = 0.0
+    with torch.cuda.stream(stream):
+        for k in range(warmup + ticks):
+            timed = k >= warmup
+            if timed:
+                ev[0].record(stream)
+            S.check(L.svgf_denoise_batch(ctx.handle, reqs[k % 2], agents, sh), "svgf_denoise_batch")
+            if timed:
+                ev[1].record(stream)
+            S.check(L.svgf_wire_encode(wire.handle, wfs[k % 2], agents, ctypes.byref(wp), sh), "svgf_wire_encode")
+            if timed:
+                ev[2].record(stream)
+            S.check(L.svgf_readback_push(rb.handle, ctypes.c_void_p(packed.data_ptr()), off, sh, ctypes.byref(seq)),
+                    "svgf_readback_push")
+            if timed:
+                ev[3].record(stream)
+                ev[3].synchronize()
+                t_den += ev[0].elapsed_time(ev[1])
+                t_enc += ev[1].elapsed_time(ev[2])
+                t_rb += ev[2].elapsed_time(ev[3])
+    t_den, t_enc, t_rb = t_den / ticks, t_enc / ticks, t_rb / ticks
+    out = {
+        "what": f"{agents} agent streams, eye sizes uniform over 8x8..256x256 (docs/AI_INTERFACE.md:354-360): "
+                "batched FP16 canonical SVGF, wire encode (fitted ACES + sRGB RGB888, depth in [0, 1]) in one launch, "
+                "one device->host push into the 3-slot pinned ring",
+        "pixels_per_tick": npx,
+        "wire_bytes_per_tick": off,
+        "tick_ms": round(t_den + t_enc + t_rb, 4),
+        "denoise_ms": round(t_den, 4),
+        "encode_ms": round(t_enc, 5),
+        "encode_mpx_per_s": round(npx / t_enc / 1e3, 1),
+        "encode_roofline": {"bound": "hbm", "algorithmic_bytes_per_px": WIRE_BYTES_PX,
+                            "achieved_GBps": round(npx * WIRE_BYTES_PX / t_enc / 1e6, 1),
+                            "frac": round(npx * WIRE_BYTES_PX / t_enc / 1e6 / HBM_PEAK_GBPS, 4)},
+        "readback_ms": round(t_rb, 4),
+        "readback_GBps": round(off / t_rb / 1e6, 2),
+    }
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+
+        col = outs[0].float().cpu().numpy()
+        g = ins[0][1]["gbuffer"].cpu().numpy()
+        reps, t0 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t0 < 1.0:
+            O.wire_encode(col, g)
+            reps += 1
+        out["cpu_encode_mpx_per_s"] = round(col.shape[0] * col.shape[1] * reps / (time.perf_counter() - t0) / 1e6, 2)
+        out["cpu_encode_what"] = f"oracle (C, 1 thread), {col.shape[1]}x{col.shape[0]} frame"
+    rb.close()
+    wire.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +254,8 @@ def main():
     ap.add_argument("--frames", type=int, default=64, help="frames per batch")
     ap.add_argument("--distinct", type=int, default=16, help="distinct input frames per stream (cycled)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--agents", type=int, default=256, help="agent streams of the agent-frame tick (0: skip)")
+    ap.add_argument("--agent-ticks", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="serving simulation length (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4000.0, help="mean Poisson arrival rate, requests/s")
@@ -388,6 +511,9 @@ def main():
                                    rate_hz=args.serve_rate, duration_s=args.serve_seconds, churn_per_s=0.05, seed=11)
         sctx.close()
 
+    # ---- agent frames: batched SVGF at eye sizes + wire encode + readback, one tick ----
+    agent = agent_report(args.agents, args.agent_ticks, args.warmup, not args.no_cpu_baseline) if args.agents > 0 else None
+
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
     cpu = None
     if not args.no_cpu_baseline:
@@ -474,6 +600,7 @@ def main():
             "requests": int(serving["completed"]), "rejected": int(serving["rejected"]),
             "mean_batch_frames": round(serving["mean_batch_frames"], 2),
             "streams_opened": int(serving["streams_opened"])},
+        "agent_frames": agent,
         "batched_other_launch": other,
         "plain_fp32": {"value": round(plain_mpx, 2), "unit": "Mpx/s", "bytes_per_px": bytes_px_32,
                        "ratio_batched_over_plain": round(batched_mpx / plain_mpx, 3)},

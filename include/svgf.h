@@ -44,7 +44,8 @@ typedef enum svgf_status {
     SVGF_ERR_CAPACITY = 6,         /* more streams / pixels than the context was created for */
     SVGF_ERR_HIP = 7,              /* a HIP runtime call failed */
     SVGF_ERR_NO_DEVICE = 8,        /* no usable gfx950 device */
-    SVGF_ERR_TIMEOUT = 9           /* a bounded wait gave up (batched scheduler) */
+    SVGF_ERR_TIMEOUT = 9,          /* a bounded wait gave up (batched scheduler) */
+    SVGF_ERR_NOT_READY = 10        /* a non-blocking query found the work still in flight */
 } svgf_status;
 
 /* Algorithm selection.
@@ -305,6 +306,57 @@ typedef struct svgf_synth_desc {
  * (normal.xyz, depth), motion RG32F. */
 SVGF_API svgf_status svgf_synth_frame(const svgf_synth_desc* desc, float* radiance, float* gbuffer,
                              float* motion);
+
+/* --- agent frames on the wire ------------------------------------------------------------
+ * The offscreen-frame layout an agent ("brain") reads (docs/AI_INTERFACE.md:346-383,
+ * TgVisionData): per frame RGB888, row-major from the top row, no row padding (3 w h bytes),
+ * and float32 depth normalised to [0, 1] (4 w h bytes). Colour is the denoised HDR output
+ * through TronGrid's display tonemap (src/postprocess.slang:47-103, 153-160: fitted ACES, then
+ * sRGB), each channel quantised to floor(255 srgb + 1/2); the display-only screen effects
+ * (bloom, chromatic aberration, grade, vignette) are not applied. Depth is
+ * (z - near) / (far - near) clamped to [0, 1]; pixels without geometry (z <= 0, not finite) read
+ * 1. One call encodes the frames of many agents in a single launch. */
+typedef struct svgf_wire svgf_wire;
+
+typedef struct svgf_wire_params {
+    float exposure;   /* linear scale before the tonemap (1: none) */
+    float depth_near; /* depth mapped to 0 (camera near plane, src/camera.hpp:99) */
+    float depth_far;  /* depth mapped to 1 (camera far plane, src/camera.hpp:100) */
+    uint32_t reserved;
+} svgf_wire_params;
+
+typedef struct svgf_wire_frame {
+    uint32_t width, height;
+    svgf_plane colour;  /* denoised output, RGBA16F or RGBA32F (one format per call) */
+    svgf_plane gbuffer; /* RGBA32F normal.xyz + linear depth (depth read from .w) */
+    void* rgb;          /* device, 3 w h bytes, 4-byte aligned */
+    void* depth;        /* device, w h floats, 16-byte aligned */
+} svgf_wire_frame;
+
+SVGF_API svgf_status svgf_wire_default_params(svgf_wire_params* out);
+SVGF_API svgf_status svgf_wire_create(int32_t device, uint32_t max_frames, svgf_wire** out);
+SVGF_API svgf_status svgf_wire_destroy(svgf_wire* wire);
+/* Enqueues the encoding of n frames on hip_stream (asynchronous). */
+SVGF_API svgf_status svgf_wire_encode(svgf_wire* wire, const svgf_wire_frame* frames, uint32_t n,
+                                      const svgf_wire_params* params, void* hip_stream);
+
+/* --- readback ring ----------------------------------------------------------------------------
+ * The agent-side copy of wire frames (docs/AI_INTERFACE.md:385-400): `slots` pinned host buffers
+ * of slot_bytes each. svgf_readback_push enqueues a device -> host copy into the next slot on
+ * hip_stream and returns its sequence number (1, 2, ...); slot (seq mod slots) is overwritten
+ * `slots` pushes later (push waits, on the host, for that slot's previous copy), so with 3 slots
+ * a reader may hold frame n while frames n + 1 and n + 2 are produced. svgf_readback_view gives
+ * the host address of a held sequence number: wait = 1 blocks until its copy is done, wait = 0
+ * returns SVGF_ERR_NOT_READY while it is in flight; an overwritten or unknown sequence number is
+ * SVGF_ERR_INVALID_ARGUMENT. */
+typedef struct svgf_readback svgf_readback;
+
+SVGF_API svgf_status svgf_readback_create(int32_t device, uint64_t slot_bytes, uint32_t slots, svgf_readback** out);
+SVGF_API svgf_status svgf_readback_destroy(svgf_readback* rb);
+SVGF_API svgf_status svgf_readback_push(svgf_readback* rb, const void* device_src, uint64_t bytes, void* hip_stream,
+                                        uint64_t* seq);
+SVGF_API svgf_status svgf_readback_view(svgf_readback* rb, uint64_t seq, int32_t wait, const void** host,
+                                        uint64_t* bytes);
 
 #ifdef __cplusplus
 }

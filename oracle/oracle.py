@@ -80,6 +80,13 @@ def lib():
         L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
         L.oracle_f16_to_f32.restype = ctypes.c_float
         L.oracle_round_f16_array.argtypes = [fp, ctypes.c_int64]
+        L.oracle_wire_thresholds.argtypes = [fp]
+        L.oracle_wire_tonemap.argtypes = [fp, ctypes.c_float, fp]
+        L.oracle_wire_codes.argtypes = [fp, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]
+        L.oracle_wire_depth.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.oracle_wire_depth.restype = ctypes.c_float
+        L.oracle_wire_encode.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                         fp, fp, ctypes.POINTER(ctypes.c_uint8), fp]
         _lib = L
     return _lib
 
@@ -167,3 +174,46 @@ def c_frame_f64(radiance, gbuffer, gbuffer_prev, motion, history_prev, moments_p
     if rc:
         raise ValueError(f"oracle_c_frame_f64 failed: {rc}")
     return out
+
+
+# ---------------------------------------------------------------------------------------------
+# agent-frame wire encoder (oracle/wire_oracle.c)
+# ---------------------------------------------------------------------------------------------
+WIRE_NEAR, WIRE_FAR = 0.1, 100.0  # camera planes, src/camera.hpp:99-100
+
+
+def wire_thresholds() -> np.ndarray:
+    """The 255 float decision thresholds of the 8-bit sRGB code (ascending)."""
+    t = np.zeros(255, np.float32)
+    lib().oracle_wire_thresholds(_fp(t))
+    return t
+
+
+def wire_codes(x) -> np.ndarray:
+    """8-bit sRGB codes of linear values (any shape)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(x.shape, np.uint8)
+    lib().oracle_wire_codes(_fp(x), x.size, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out
+
+
+def wire_tonemap(rgb, exposure: float = 1.0) -> np.ndarray:
+    """Fitted ACES of one (3,) colour (float arithmetic), in [0, 1]."""
+    src = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().oracle_wire_tonemap(_fp(src), float(exposure), _fp(out))
+    return out
+
+
+def wire_encode(colour, gbuffer, exposure: float = 1.0, near: float = WIRE_NEAR, far: float = WIRE_FAR):
+    """colour (h, w, 4) and gbuffer (h, w, 4) float32 -> (rgb (h, w, 3) uint8, depth (h, w) float32)."""
+    colour = _f32(colour)
+    gbuffer = _f32(gbuffer)
+    h, w = colour.shape[:2]
+    rgb = np.zeros((h, w, 3), np.uint8)
+    depth = np.zeros((h, w), np.float32)
+    rc = lib().oracle_wire_encode(w, h, float(exposure), float(near), float(far), _fp(colour), _fp(gbuffer),
+                                  rgb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _fp(depth))
+    if rc:
+        raise ValueError(f"oracle_wire_encode failed: {rc}")
+    return rgb, depth

@@ -81,6 +81,16 @@ int oracle_c_frame_f64_mt(int w, int h, const oracle_params* p, int history_vali
                           double* history_curr, double* moments_curr, double* denoised,
                           double* dbg_temporal, double* dbg_variance, int threads);
 
+/* Agent-frame wire encoder (wire_oracle.c): docs/AI_INTERFACE.md:346-383 layout, colour through
+ * postprocess.slang's fitted ACES + sRGB, quantised by decision thresholds. */
+void oracle_wire_thresholds(float* t255);
+void oracle_wire_tonemap(const float* rgb, float exposure, float* out3);
+void oracle_wire_codes(const float* x, int64_t n, uint8_t* out); /* 8-bit sRGB code of linear x */
+float oracle_wire_depth(float z, float near_z, float far_z);
+/* colour, gbuffer: w*h*4 float; rgb: w*h*3 bytes; depth: w*h float. Returns 0 on success. */
+int oracle_wire_encode(int w, int h, float exposure, float near_z, float far_z, const float* colour,
+                       const float* gbuffer, uint8_t* rgb, float* depth);
+
 #ifdef __cplusplus
 }
 #endif

@@ -188,3 +188,28 @@ def test_binding_flags_match_header():
     assert defs["PROFILE_STAGES"] == S.PROFILE_STAGES
     assert defs["BATCH_PERSISTENT"] == S.BATCH_PERSISTENT
     assert defs["FLAG_COMPENSATED_MOMENTS"] == S.FLAG_COMPENSATED
+
+
+def test_wire_and_readback_argument_checks_without_gpu():
+    """Host-side checks of the wire encoder and readback ring (no kernel launched)."""
+    L = S.lib()
+    p = S.wire_default_params()
+    assert (p.exposure, round(p.depth_near, 6), round(p.depth_far, 6)) == (1.0, 0.1, 100.0)
+    assert L.svgf_wire_default_params(None) == 1
+    h = ctypes.c_void_p()
+    assert L.svgf_wire_create(0, 0, ctypes.byref(h)) == 1       # no frames
+    assert L.svgf_wire_create(0, 1 << 20, ctypes.byref(h)) == 1  # over the frame limit
+    assert L.svgf_wire_encode(None, None, 0, ctypes.byref(p), None) == 1
+    assert L.svgf_readback_create(0, 0, 3, ctypes.byref(h)) == 1
+    assert L.svgf_readback_create(0, 64, 1, ctypes.byref(h)) == 1  # a ring needs two slots at least
+    assert L.svgf_readback_push(None, None, 8, None, None) == 1
+    assert L.svgf_readback_view(None, 1, 1, None, None) == 1
+    try:
+        import torch
+
+        gpu = torch.cuda.is_available()
+    except ImportError:
+        gpu = False
+    if not gpu:
+        assert L.svgf_wire_create(0, 16, ctypes.byref(h)) == 8
+        assert L.svgf_readback_create(0, 64, 3, ctypes.byref(h)) == 8

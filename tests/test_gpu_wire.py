@@ -1,0 +1,164 @@
+"""GPU tests of the agent-frame wire encoder and the readback ring, through the C ABI.
+
+The encoder's output is bytes and exactly computed floats: it must equal the oracle
+(oracle/wire_oracle.c) bit for bit, on the agent sizes of docs/AI_INTERFACE.md:354-360
+(8x8 .. 256x256) and on ragged sizes that end inside a 4-pixel group.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+SIZES = [(8, 8), (32, 32), (64, 64), (128, 128), (256, 256), (1, 1), (3, 5), (17, 9), (255, 3), (96, 33)]
+
+
+@pytest.fixture(scope="module")
+def S():
+    import svgf_amd
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    svgf_amd.lib()
+    return svgf_amd
+
+
+def _inputs(S, w, h, seed):
+    d = S.synth_frame(w, h, seed=seed, frame=0, noise=1, pan_px=0.5)
+    col = d["radiance"].copy()
+    g = d["gbuffer"].copy()
+    rng = np.random.default_rng(seed)
+    # HDR extremes, black, and pixels without geometry
+    k = max(1, w * h // 16)
+    idx = rng.integers(0, w * h, k)
+    col.reshape(-1, 4)[idx[: k // 2 + 1], :3] *= 400.0
+    col.reshape(-1, 4)[idx[k // 2:], :3] = 0.0
+    zidx = rng.integers(0, w * h, k)
+    g.reshape(-1, 4)[zidx, 3] = np.float32([0.0, -2.0, np.inf, np.nan, 150.0, 0.05])[np.arange(k) % 6]
+    return col, g
+
+
+def _encode_all(S, sizes, dtype, params=None, seed0=0):
+    dev = torch.device("cuda")
+    items, host = [], []
+    for k, (w, h) in enumerate(sizes):
+        col, g = _inputs(S, w, h, seed0 + k)
+        colour = torch.from_numpy(col).to(dev).to(dtype)
+        items.append({"colour": colour, "gbuffer": torch.from_numpy(g).to(dev),
+                      "rgb": torch.empty(3 * w * h + 4, dtype=torch.uint8, device=dev),
+                      "depth": torch.empty(w * h + 4, dtype=torch.float32, device=dev)})
+        host.append((colour.float().cpu().numpy(), g))
+    with S.Wire(max_frames=len(sizes)) as wire:
+        wire.encode(items, params)
+        torch.cuda.synchronize()
+    return items, host
+
+
+@pytest.mark.parametrize("dtype", ["fp16", "fp32"])
+def test_wire_matches_oracle_bit_for_bit(S, oracle_lib, dtype):
+    O = oracle_lib
+    dt = torch.float16 if dtype == "fp16" else torch.float32
+    items, host = _encode_all(S, SIZES, dt)
+    for (w, h), it, (col, g) in zip(SIZES, items, host):
+        rgb, depth = O.wire_encode(col, g)
+        got_rgb = it["rgb"][: 3 * w * h].cpu().numpy().reshape(h, w, 3)
+        got_depth = it["depth"][: w * h].cpu().numpy().reshape(h, w)
+        assert np.array_equal(got_rgb, rgb), (w, h, int((got_rgb != rgb).sum()))
+        assert np.array_equal(got_depth.view(np.uint32), depth.view(np.uint32)), (w, h)
+
+
+def test_wire_params_and_no_overrun(S, oracle_lib):
+    """Exposure and depth planes reach the kernel; the bytes just past each frame's planes stay
+    untouched (the tail groups write byte by byte)."""
+    O = oracle_lib
+    p = S.wire_default_params()
+    p.exposure, p.depth_near, p.depth_far = 0.25, 1.0, 40.0
+    dev = torch.device("cuda")
+    items, host = [], []
+    for k, (w, h) in enumerate(SIZES):
+        col, g = _inputs(S, w, h, 100 + k)
+        rgb = torch.full((3 * w * h + 8,), 0xA5, dtype=torch.uint8, device=dev)
+        dep = torch.full((w * h + 8,), -7.0, dtype=torch.float32, device=dev)
+        items.append({"colour": torch.from_numpy(col).to(dev), "gbuffer": torch.from_numpy(g).to(dev), "rgb": rgb,
+                      "depth": dep})
+        host.append((col, g))
+    with S.Wire(max_frames=len(SIZES)) as wire:
+        wire.encode(items, p)
+        torch.cuda.synchronize()
+    for (w, h), it, (col, g) in zip(SIZES, items, host):
+        rgb, depth = O.wire_encode(col, g, exposure=0.25, near=1.0, far=40.0)
+        r = it["rgb"].cpu().numpy()
+        d = it["depth"].cpu().numpy()
+        assert np.array_equal(r[: 3 * w * h].reshape(h, w, 3), rgb)
+        assert np.array_equal(d[: w * h].reshape(h, w), depth)
+        assert np.all(r[3 * w * h:] == 0xA5) and np.all(d[w * h:] == -7.0), (w, h)
+
+
+def test_readback_ring_semantics(S):
+    dev = torch.device("cuda")
+    n = 4096
+    srcs = [torch.full((n,), i + 1, dtype=torch.uint8, device=dev) for i in range(5)]
+    torch.cuda.synchronize()
+    with S.Readback(n, slots=3) as rb:
+        seqs = [rb.push(s.data_ptr(), n) for s in srcs]
+        assert seqs == [1, 2, 3, 4, 5]
+        for q in (3, 4, 5):
+            v = rb.view(q)
+            assert v.size == n and np.all(v == q)
+        for q in (1, 2, 9):  # overwritten, overwritten, never pushed
+            with pytest.raises(S.SvgfError) as e:
+                rb.view(q)
+            assert e.value.status == 1
+        with pytest.raises(S.SvgfError) as e:
+            rb.push(srcs[0].data_ptr(), n + 1)  # larger than a slot
+        assert e.value.status == 6
+
+
+def test_agent_frames_end_to_end(S, oracle_lib):
+    """Resident agent streams denoised in a batch (FP16), encoded to the wire in one launch and
+    read back through the ring into one host buffer: each agent's bytes equal the oracle's
+    encoding of the GPU's denoised output."""
+    O = oracle_lib
+    dev = torch.device("cuda")
+    sizes = [(8, 8), (32, 32), (64, 64), (128, 128), (256, 256)] * 3
+    npx = sum(w * h for w, h in sizes)
+    wire_bytes = 7 * npx
+    rgb_off, dep_off, off = [], [], 0
+    for w, h in sizes:  # one packed wire buffer: every agent's rgb, then (16-byte aligned) depth
+        rgb_off.append(off)
+        off += (3 * w * h + 15) // 16 * 16
+    for w, h in sizes:
+        dep_off.append(off)
+        off += 4 * w * h
+    packed = torch.empty(off, dtype=torch.uint8, device=dev)
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=256, max_height=256, max_streams=len(sizes),
+                   max_batch_frames=len(sizes), max_batch_pixels=npx) as ctx, \
+            S.Wire(max_frames=len(sizes)) as wire, S.Readback(off, slots=3) as rb:
+        sids = [ctx.stream_open(w, h) for w, h in sizes]
+        outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes]
+        gbs = []
+        for f in range(2):
+            items = []
+            gbs = []
+            for i, (w, h) in enumerate(sizes):
+                d = S.synth_frame(w, h, seed=50 * i + f, stream=i, frame=f, pan_px=0.5)
+                gb = torch.from_numpy(d["gbuffer"]).to(dev)
+                gbs.append(gb)
+                items.append({"stream_id": sids[i], "radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                              "gbuffer": gb, "motion": torch.from_numpy(d["motion"]).to(dev), "denoised": outs[i]})
+            ctx.denoise_batch(items)
+        wframes = []
+        for i, (w, h) in enumerate(sizes):
+            wframes.append({"colour": outs[i], "gbuffer": gbs[i],
+                            "rgb": packed[rgb_off[i]: rgb_off[i] + 3 * w * h],
+                            "depth": packed[dep_off[i]: dep_off[i] + 4 * w * h].view(torch.float32)})
+        wire.encode(wframes)
+        seq = rb.push(packed.data_ptr(), off)
+        host = rb.view(seq).copy()
+        assert wire_bytes <= off
+        for i, (w, h) in enumerate(sizes):
+            rgb, depth = O.wire_encode(outs[i].float().cpu().numpy(), gbs[i].cpu().numpy())
+            assert np.array_equal(host[rgb_off[i]: rgb_off[i] + 3 * w * h].reshape(h, w, 3), rgb), i
+            assert np.array_equal(host[dep_off[i]: dep_off[i] + 4 * w * h].view(np.float32).reshape(h, w), depth), i

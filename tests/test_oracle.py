@@ -507,3 +507,84 @@ def test_canonical_crop_interior_equals_full_frame(compensated):
         if x0 > 0:
             a = full["denoised"][y0:y0 + c, x0:x0 + c][ys, :MARGIN // 4]
             assert not np.array_equal(a, ref["denoised"][ys, :MARGIN // 4])
+
+
+# ------------------------------------------------------------------------------------------
+# agent-frame wire encoder (oracle/wire_oracle.c)
+# ------------------------------------------------------------------------------------------
+def _srgb64(x):
+    x = np.asarray(x, np.float64)
+    return np.where(x <= 0.0031308, x * 12.92, 1.055 * np.power(np.maximum(x, 0.0), 1.0 / 2.4) - 0.055)
+
+
+def test_wire_thresholds_are_the_smallest_floats_reaching_each_code():
+    t = O.wire_thresholds()
+    assert np.all(np.diff(t) > 0)
+    k = np.arange(1, 256, dtype=np.float64)
+    target = (k - 0.5) / 255.0
+    below = np.nextafter(t, np.float32(0.0))
+    assert np.all(_srgb64(t) >= target)
+    assert np.all(_srgb64(below) < target)
+
+
+def test_wire_codes_equal_rounded_srgb():
+    """code(x) = floor(255 srgb(x) + 1/2) in double, on random values, every threshold and its
+    float neighbours, 0, 1 and values outside [0, 1]."""
+    rng = np.random.default_rng(7)
+    t = O.wire_thresholds()
+    x = np.concatenate([rng.random(200_000, dtype=np.float32), t, np.nextafter(t, np.float32(0)),
+                        np.nextafter(t, np.float32(2)), np.float32([0.0, 1.0, 1e-9, 0.0031308, 0.5])])
+    want = np.floor(255.0 * _srgb64(np.clip(x, 0, 1)) + 0.5).astype(np.uint8)
+    assert np.array_equal(O.wire_codes(x), want)
+    assert list(O.wire_codes(np.float32([-1.0, 2.0, np.inf]))) == [0, 255, 255]
+
+
+def test_wire_tonemap_grey_axis_is_the_fitted_curve():
+    """The input matrix has unit row sums (postprocess.slang:49-53), so a grey input is the same
+    grey in AP1; output channel r is then the RRT+ODT fit of the grey level times row r's sum of
+    the output matrix (1, 1 and 0.99999 for the published coefficients, :56-60). A transposed
+    matrix (column sums 0.70 / 1.40 / 0.92 in, 1.50 / 0.50 / 1.00 out) or a wrong fit
+    coefficient breaks this."""
+    def fit(v):
+        return (v * (v + 0.0245786) - 0.000090537) / (v * (0.983729 * v + 0.4329510) + 0.238081)
+
+    out_row_sums = np.array([1.60475 - 0.53108 - 0.07367, -0.10208 + 1.10813 - 0.00605, -0.00327 - 0.07276 + 1.07602])
+    for g in [0.0, 1e-3, 0.01, 0.18, 0.5, 1.0, 2.0, 8.0, 64.0, 1e4]:
+        out = O.wire_tonemap([g, g, g])
+        want = np.clip(fit(g) * out_row_sums, 0.0, 1.0)
+        assert np.allclose(out, want, rtol=2e-6, atol=2e-7), (g, out, want)
+    # a saturated primary stays within [0, 1]; exposure scales the input
+    assert np.all((O.wire_tonemap([50.0, 0.0, 0.0]) >= 0) & (O.wire_tonemap([50.0, 0.0, 0.0]) <= 1))
+    assert np.array_equal(O.wire_tonemap([0.1, 0.2, 0.3], exposure=2.0), O.wire_tonemap([0.2, 0.4, 0.6]))
+
+
+def test_wire_tonemap_is_monotonic_along_grey():
+    g = np.float32(np.geomspace(1e-4, 1e3, 400))
+    y = np.array([O.wire_tonemap([v, v, v])[0] for v in g])
+    assert np.all(np.diff(y) >= 0)
+
+
+def test_wire_depth_normalisation():
+    near, far = O.WIRE_NEAR, O.WIRE_FAR
+    L = O.lib()
+    assert L.oracle_wire_depth(np.float32(near), near, far) == 0.0
+    assert L.oracle_wire_depth(np.float32(far), near, far) == 1.0
+    assert L.oracle_wire_depth(np.float32(500.0), near, far) == 1.0
+    assert L.oracle_wire_depth(np.float32(0.05), near, far) == 0.0
+    for z in (0.0, -3.0, float("nan"), float("inf")):
+        assert L.oracle_wire_depth(z, near, far) == 1.0  # no geometry reads as far
+    assert abs(L.oracle_wire_depth(np.float32(50.05), near, far) - 0.5) < 1e-6
+
+
+def test_wire_encode_layout():
+    """rgb[i] of pixel (x, y) sits at byte 3 (y w + x): rows top to bottom, no padding."""
+    rng = np.random.default_rng(1)
+    h, w = 5, 7
+    col = rng.random((h, w, 4), dtype=np.float32) * 3
+    g = np.zeros((h, w, 4), np.float32)
+    g[..., 3] = 1.0 + rng.random((h, w), dtype=np.float32) * 50
+    rgb, depth = O.wire_encode(col, g)
+    for y, x in [(0, 0), (2, 3), (4, 6)]:
+        assert np.array_equal(rgb[y, x], O.wire_codes(O.wire_tonemap(col[y, x, :3])))
+        assert depth[y, x] == O.lib().oracle_wire_depth(g[y, x, 3], O.WIRE_NEAR, O.WIRE_FAR)
+    assert rgb.tobytes()[3 * (2 * w + 3):3 * (2 * w + 3) + 3] == rgb[2, 3].tobytes()

@@ -344,6 +344,7 @@ const char* svgf_status_str(svgf_status s)
     case SVGF_ERR_HIP: return "HIP runtime error";
     case SVGF_ERR_NO_DEVICE: return "no gfx950 device";
     case SVGF_ERR_TIMEOUT: return "timeout";
+    case SVGF_ERR_NOT_READY: return "not ready";
     }
     return "unknown status";
 }

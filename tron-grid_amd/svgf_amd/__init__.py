@@ -36,6 +36,7 @@ BATCH_PERSISTENT = 0x4
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "unsupported", 3: "out of device memory", 4: "over HBM budget",
     5: "no such stream", 6: "over context capacity", 7: "HIP runtime error", 8: "no gfx950 device", 9: "timeout",
+    10: "not ready",
 }
 
 
@@ -96,6 +97,16 @@ class Frame(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("stream_id", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("radiance", Plane),
                 ("gbuffer", Plane), ("motion", Plane), ("denoised", Plane)]
+
+
+class WireParams(ctypes.Structure):
+    _fields_ = [("exposure", ctypes.c_float), ("depth_near", ctypes.c_float), ("depth_far", ctypes.c_float),
+                ("reserved", ctypes.c_uint32)]
+
+
+class WireFrame(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("colour", Plane), ("gbuffer", Plane),
+                ("rgb", ctypes.c_void_p), ("depth", ctypes.c_void_p)]
 
 
 class StreamState(ctypes.Structure):
@@ -194,6 +205,14 @@ def lib():
             "svgf_server_poll": [vp, P(Completion), ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_uint32],
             "svgf_server_drain": [vp],
             "svgf_server_simulate": [vp, P(SimDesc), P(SimResult)],
+            "svgf_wire_default_params": [P(WireParams)],
+            "svgf_wire_create": [ctypes.c_int32, ctypes.c_uint32, P(vp)],
+            "svgf_wire_destroy": [vp],
+            "svgf_wire_encode": [vp, P(WireFrame), ctypes.c_uint32, P(WireParams), vp],
+            "svgf_readback_create": [ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint32, P(vp)],
+            "svgf_readback_destroy": [vp],
+            "svgf_readback_push": [vp, vp, ctypes.c_uint64, vp, P(ctypes.c_uint64)],
+            "svgf_readback_view": [vp, ctypes.c_uint64, ctypes.c_int32, P(vp), P(ctypes.c_uint64)],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args
@@ -463,3 +482,92 @@ def _hip():
         _hip_lib = ctypes.CDLL("libamdhip64.so")
         _hip_lib.hipMemcpy2D.restype = ctypes.c_int
     return _hip_lib
+
+
+# ---------------------------------------------------------------------------------------------
+# agent frames on the wire (RGB888 + normalised depth) and the readback ring
+# ---------------------------------------------------------------------------------------------
+def wire_default_params() -> WireParams:
+    p = WireParams()
+    check(lib().svgf_wire_default_params(ctypes.byref(p)), "svgf_wire_default_params")
+    return p
+
+
+class Wire:
+    """Batched wire encoder: denoised colour + G-buffer depth -> RGB888 + depth in [0, 1]."""
+
+    def __init__(self, max_frames=256, device=0):
+        self.handle = ctypes.c_void_p()
+        check(lib().svgf_wire_create(device, max_frames, ctypes.byref(self.handle)), "svgf_wire_create")
+
+    def close(self):
+        if self.handle:
+            check(lib().svgf_wire_destroy(self.handle), "svgf_wire_destroy")
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def encode(self, items, params=None, stream=None):
+        """items: dicts with colour (h, w, 4) fp16/fp32, gbuffer (h, w, 4) fp32, rgb (uint8 tensor of
+        3 w h bytes), depth (float32 tensor of w h values), all on the device."""
+        torch = _torch()
+        arr = (WireFrame * len(items))()
+        for k, it in enumerate(items):
+            h, w = it["colour"].shape[:2]
+            f = arr[k]
+            f.width, f.height = w, h
+            f.colour = plane_of(it["colour"], FMT_RGBA16F if it["colour"].dtype == torch.float16 else FMT_RGBA32F)
+            f.gbuffer = plane_of(it["gbuffer"], FMT_RGBA32F)
+            f.rgb = it["rgb"].data_ptr()
+            f.depth = it["depth"].data_ptr()
+        p = params or wire_default_params()
+        check(lib().svgf_wire_encode(self.handle, arr, len(items), ctypes.byref(p), _stream_handle(stream)),
+              "svgf_wire_encode")
+
+
+class Readback:
+    """Pinned host ring of `slots` buffers (svgf_readback_*)."""
+
+    def __init__(self, slot_bytes, slots=3, device=0):
+        self.handle = ctypes.c_void_p()
+        check(lib().svgf_readback_create(device, slot_bytes, slots, ctypes.byref(self.handle)), "svgf_readback_create")
+
+    def close(self):
+        if self.handle:
+            check(lib().svgf_readback_destroy(self.handle), "svgf_readback_destroy")
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def push(self, device_ptr, nbytes, stream=None) -> int:
+        seq = ctypes.c_uint64()
+        check(lib().svgf_readback_push(self.handle, ctypes.c_void_p(device_ptr), nbytes, _stream_handle(stream),
+                                       ctypes.byref(seq)), "svgf_readback_push")
+        return seq.value
+
+    def view(self, seq, wait=True):
+        """numpy uint8 view of the host copy of push `seq` (valid until the slot is reused)."""
+        ptr, n = ctypes.c_void_p(), ctypes.c_uint64()
+        check(lib().svgf_readback_view(self.handle, seq, 1 if wait else 0, ctypes.byref(ptr), ctypes.byref(n)),
+              "svgf_readback_view")
+        return np.ctypeslib.as_array((ctypes.c_uint8 * n.value).from_address(ptr.value))
