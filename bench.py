@@ -131,8 +131,10 @@ WIRE_BYTES_PX = 8 + 4 + 3 + 4  # fp16 colour + depth (.w of the G-buffer) in; RG
 
 def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
     """One agent tick: `agents` resident streams at brain-requested eye sizes are denoised in one
-    batch (FP16 canonical), encoded to the wire (RGB888 + depth) in one launch, and the packed
-    wire buffer is pushed into the 3-slot pinned readback ring."""
+    batch (FP16 canonical); the next frame's composite (shading terms + reflections reprojected
+    into the just-denoised output) runs in one launch; the composited frames are encoded to the
+    wire (RGB888 + depth) in one launch, and the packed wire buffer is pushed into the 3-slot
+    pinned readback ring."""
     import numpy as np
     import torch
 
@@ -161,6 +163,21 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
                        "motion": torch.from_numpy(d["motion"]).to(dev)})
         ins.append(fr)
     outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes]
+    from svgf_amd.scenes import composite_scene
+
+    cplanes = ("emissive", "fresnel", "reflection", "reflection_colour", "reflection_base")
+    citems = []
+    for i, (w, h) in enumerate(sizes):
+        d = composite_scene(w, h, 3000 + i, half=True)
+        it = {"prev_view_projection": d["m"], "history": outs[i],
+              "out": torch.empty((h, w, 4), dtype=torch.float16, device=dev)}
+        for k in cplanes:
+            t = torch.from_numpy(d[k]).to(dev)
+            it[k] = t if k == "reflection" else t.half()
+        citems.append(it)
+    carr = S.Composite.frames(citems)
+    comp = S.Composite(max_frames=agents)
+    cp = S.composite_default_params()
     ctx = S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=256, max_height=256, max_streams=agents,
                     max_batch_frames=agents, max_batch_pixels=npx)
     wire = S.Wire(max_frames=agents)
@@ -180,7 +197,7 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
             r.denoised = S.plane_of(outs[i], S.FMT_RGBA16F)
             q = warr[i]
             q.width, q.height = w, h
-            q.colour = S.plane_of(outs[i], S.FMT_RGBA16F)
+            q.colour = S.plane_of(citems[i]["out"], S.FMT_RGBA16F)
             q.gbuffer = S.plane_of(ins[i][f]["gbuffer"], S.FMT_RGBA32F)
             q.rgb = packed.data_ptr() + rgb_off[i]
             q.depth = packed.data_ptr() + dep_off[i]
@@ -190,8 +207,8 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
     sh = ctypes.c_void_p(stream.cuda_stream)
     wp = S.wire_default_params()
     seq = ctypes.c_uint64()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    t_den = t_enc = t_rb = 0.0
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    t_den = t_cmp = t_enc = t_rb = 0.0
     with torch.cuda.stream(stream):
         for k in range(warmup + ticks):
             timed = k >= warmup
@@ -200,26 +217,31 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
             S.check(L.svgf_denoise_batch(ctx.handle, reqs[k % 2], agents, sh), "svgf_denoise_batch")
             if timed:
                 ev[1].record(stream)
-            S.check(L.svgf_wire_encode(wire.handle, wfs[k % 2], agents, ctypes.byref(wp), sh), "svgf_wire_encode")
+            S.check(L.svgf_composite_run(comp.handle, carr, agents, ctypes.byref(cp), sh), "svgf_composite_run")
             if timed:
                 ev[2].record(stream)
+            S.check(L.svgf_wire_encode(wire.handle, wfs[k % 2], agents, ctypes.byref(wp), sh), "svgf_wire_encode")
+            if timed:
+                ev[3].record(stream)
             S.check(L.svgf_readback_push(rb.handle, ctypes.c_void_p(packed.data_ptr()), off, sh, ctypes.byref(seq)),
                     "svgf_readback_push")
             if timed:
-                ev[3].record(stream)
-                ev[3].synchronize()
+                ev[4].record(stream)
+                ev[4].synchronize()
                 t_den += ev[0].elapsed_time(ev[1])
-                t_enc += ev[1].elapsed_time(ev[2])
-                t_rb += ev[2].elapsed_time(ev[3])
-    t_den, t_enc, t_rb = t_den / ticks, t_enc / ticks, t_rb / ticks
+                t_cmp += ev[1].elapsed_time(ev[2])
+                t_enc += ev[2].elapsed_time(ev[3])
+                t_rb += ev[3].elapsed_time(ev[4])
+    t_den, t_cmp, t_enc, t_rb = t_den / ticks, t_cmp / ticks, t_enc / ticks, t_rb / ticks
     out = {
         "what": f"{agents} agent streams, eye sizes uniform over 8x8..256x256 (docs/AI_INTERFACE.md:354-360): "
-                "batched FP16 canonical SVGF, wire encode (fitted ACES + sRGB RGB888, depth in [0, 1]) in one launch, "
+                "batched FP16 canonical SVGF, consumer composite over the denoised output in one launch, wire encode (fitted ACES + sRGB RGB888, depth in [0, 1]) in one launch, "
                 "one device->host push into the 3-slot pinned ring",
         "pixels_per_tick": npx,
         "wire_bytes_per_tick": off,
-        "tick_ms": round(t_den + t_enc + t_rb, 4),
+        "tick_ms": round(t_den + t_cmp + t_enc + t_rb, 4),
         "denoise_ms": round(t_den, 4),
+        "composite_ms": round(t_cmp, 5),
         "encode_ms": round(t_enc, 5),
         "encode_mpx_per_s": round(npx / t_enc / 1e3, 1),
         "encode_roofline": {"bound": "hbm", "algorithmic_bytes_per_px": WIRE_BYTES_PX,
@@ -242,7 +264,95 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
         out["cpu_encode_what"] = f"oracle (C, 1 thread), {col.shape[1]}x{col.shape[0]} frame"
     rb.close()
     wire.close()
+    comp.close()
     ctx.close()
+    return out
+
+
+# Consumer composite (src/mesh.slang:1007, 1161-1174, 1191), algorithmic bytes per pixel: every
+# pixel reads history, emissive, F_view (rgb planes) and the reflection (RGBA32F) and writes the
+# output; a reflection with a colour (kind 1 or 2) reads reflection_colour; a reprojected hit
+# reads one history pixel, a miss reads reflection_base instead.
+def composite_bytes(scene, rgb_bytes):
+    import numpy as np
+
+    kind = scene["reflection"][..., 3]
+    hit = (kind == 2.0) & (scene["target"][..., 0] >= 0)
+    miss = (kind == 2.0) & (scene["target"][..., 0] == -1)
+    coloured = (kind == 1.0) | (kind == 2.0)
+    npx = kind.size
+    return int(npx * (4 * rgb_bytes + 16) + rgb_bytes * (int(coloured.sum()) + int(hit.sum()) + int(miss.sum())))
+
+
+def composite_report(reps: int, cpu: bool) -> dict:
+    """The reader of the denoised output as one launch: 1080p and 4K frames, FP16 (rgb planes
+    rgba16f, reflection rgba32f) and FP32, and the 64-frame batch mix in one call. Scene layout:
+    top 30 % of rows no reflection, next 15 % ready reflected colour, the rest a reflective floor
+    whose hits reproject near the mirrored pixel (5 % off screen, 3 % behind the camera); the
+    _random case draws every pixel's kind and target pixel at random (40 % in-frame hits, the
+    worst case for the history gather)."""
+    import numpy as np
+    import torch
+
+    import svgf_amd as S
+    from svgf_amd.scenes import composite_scene
+
+    planes = ("history", "emissive", "fresnel", "reflection", "reflection_colour", "reflection_base")
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    L = S.lib()
+    cp = S.composite_default_params()
+    rng = np.random.default_rng(0)
+    mix = [RESOLUTIONS[i] for i in rng.integers(0, len(RESOLUTIONS), 64)]
+    cases = [("1920x1080_fp16", [(1920, 1080)], True, True), ("3840x2160_fp16", [(3840, 2160)], True, True),
+             ("3840x2160_fp32", [(3840, 2160)], False, True), ("3840x2160_fp16_random", [(3840, 2160)], True, False),
+             ("mix64_fp16", mix, True, True)]
+    out = {"what": composite_report.__doc__.split("\n")[0].strip() + " " + " ".join(
+        ln.strip() for ln in composite_report.__doc__.split("\n")[1:]).strip()}
+    comp = S.Composite(max_frames=64)
+    for name, sizes, half, coherent in cases:
+        scenes = [composite_scene(w, h, 900 + k, half=half, coherent=coherent) for k, (w, h) in enumerate(sizes)]
+        items = []
+        for d in scenes:
+            h, w = d["history"].shape[:2]
+            it = {"prev_view_projection": d["m"]}
+            for k in planes:
+                t = torch.from_numpy(d[k]).to(dev)
+                it[k] = t.half() if half and k != "reflection" else t
+            it["out"] = torch.empty((h, w, 4), dtype=torch.float16 if half else torch.float32, device=dev)
+            items.append(it)
+        arr = S.Composite.frames(items)
+        nbytes = sum(composite_bytes(d, 8 if half else 16) for d in scenes)
+        npx = sum(w * h for w, h in sizes)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            for _ in range(10):
+                S.check(L.svgf_composite_run(comp.handle, arr, len(arr), ctypes.byref(cp), sh), "svgf_composite_run")
+            e0.record(stream)
+            for _ in range(reps):
+                S.check(L.svgf_composite_run(comp.handle, arr, len(arr), ctypes.byref(cp), sh), "svgf_composite_run")
+            e1.record(stream)
+            e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbps = nbytes / ms / 1e6
+        out[name] = {"frames": len(sizes), "pixels": npx, "ms": round(ms, 5), "mpx_per_s": round(npx / ms / 1e3, 1),
+                     "algorithmic_bytes": nbytes, "bytes_per_px": round(nbytes / npx, 2),
+                     "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+        if cpu and name == "1920x1080_fp16":
+            sys.path.insert(0, str(ROOT / "oracle"))
+            import oracle as O
+
+            d = scenes[0]
+            r, t0 = 0, time.perf_counter()
+            while r < 3 or time.perf_counter() - t0 < 1.0:
+                O.composite(d["m"], *[d[k] for k in planes], store=O.STORE_F16)
+                r += 1
+            out["cpu_1920x1080"] = {"mpx_per_s": round(1920 * 1080 * r / (time.perf_counter() - t0) / 1e6, 2),
+                                    "cores": 1, "kind": "port", "what": "oracle (C, 1 thread)"}
+        del items, arr
+    comp.close()
     return out
 
 
@@ -256,6 +366,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--agents", type=int, default=256, help="agent streams of the agent-frame tick (0: skip)")
     ap.add_argument("--agent-ticks", type=int, default=20)
+    ap.add_argument("--composite-reps", type=int, default=50,
+                    help="launches timed per consumer-composite case (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="serving simulation length (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4000.0, help="mean Poisson arrival rate, requests/s")
@@ -264,9 +376,9 @@ def main():
                     help="HBM bytes per a-trous launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE (reported as is)")
     ap.add_argument("--no-ref720", dest="ref720", action="store_false",
                     help="skip the REFERENCE-mode 1280x720 timing (the reference's published case)")
-    ap.add_argument("--report", choices=["precision", "copy"], default=None,
+    ap.add_argument("--report", choices=["precision", "copy", "composite"], default=None,
                     help="precision: error of the GPU forms vs the FP64 oracle on 16-frame 1080p / 4K sequences; "
-                         "copy: one 1 GiB device copy (counter calibration)")
+                         "copy: one 1 GiB device copy (counter calibration); composite: the consumer composite only")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched form (for profiler runs)")
     ap.add_argument("--launch", choices=["persistent", "per_stage"], default="per_stage",
@@ -277,6 +389,9 @@ def main():
     ap.add_argument("--report-threads", type=int, default=16)
     args = ap.parse_args()
 
+    if args.report == "composite":
+        print(json.dumps({"consumer_composite": composite_report(args.composite_reps, not args.no_cpu_baseline)}))
+        return
     if args.report == "precision":
         print(json.dumps({"precision": precision_report(args.report_frames, args.report_threads, args.report_scale)}))
         return
@@ -514,6 +629,9 @@ def main():
     # ---- agent frames: batched SVGF at eye sizes + wire encode + readback, one tick ----
     agent = agent_report(args.agents, args.agent_ticks, args.warmup, not args.no_cpu_baseline) if args.agents > 0 else None
 
+    # ---- consumer composite: the reader of the denoised output ----
+    composite = composite_report(args.composite_reps, not args.no_cpu_baseline) if args.composite_reps > 0 else None
+
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
     cpu = None
     if not args.no_cpu_baseline:
@@ -601,6 +719,7 @@ def main():
             "mean_batch_frames": round(serving["mean_batch_frames"], 2),
             "streams_opened": int(serving["streams_opened"])},
         "agent_frames": agent,
+        "consumer_composite": composite,
         "batched_other_launch": other,
         "plain_fp32": {"value": round(plain_mpx, 2), "unit": "Mpx/s", "bytes_per_px": bytes_px_32,
                        "ratio_batched_over_plain": round(batched_mpx / plain_mpx, 3)},

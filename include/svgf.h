@@ -358,6 +358,54 @@ SVGF_API svgf_status svgf_readback_push(svgf_readback* rb, const void* device_sr
 SVGF_API svgf_status svgf_readback_view(svgf_readback* rb, uint64_t seq, int32_t wait, const void** host,
                                         uint64_t* bytes);
 
+/* --- consumer composite -----------------------------------------------------------------------
+ * The reader of the denoised output: TronGrid's mesh pass shows the previous frame's denoised
+ * lighting at the same pixel (src/mesh.slang:1007) and, for a reflection ray that hit a
+ * non-emissive surface, at the pixel the hit position reprojects to through the previous
+ * frame's view-projection (mesh.slang:1161-1174), then composites (mesh.slang:1191):
+ *   clip = M (hit, 1); clip.w > 0 and px = int2((clip.xy / clip.w (0.5, -0.5) + 0.5) (W, H))
+ *     inside the frame: hit_indirect = history[px].rgb, else hit_base * ambient;
+ *   reflected = hit_emissive + hit_indirect;
+ *   out = (history[pixel].rgb + emissive) + reflected * F_view, alpha 1.
+ * int2() truncates toward zero (a product in (-1, 0) lands on column/row 0); a NaN or
+ * out-of-range product is off screen. Arithmetic is float, one rounding per operation, matrix
+ * rows summed left to right: ((m0 x + m1 y) + m2 z) + m3.
+ * Per pixel the caller passes what the mesh pass holds besides the history: emissive, F_view and
+ * the reflection, whose kind is reflection.w: 2 = reproject (reflection.xyz = hit position in
+ * world space, reflection_colour = hit emissive, reflection_base = hit base colour); 1 = a
+ * reflected colour that needs no lookup (sky miss, emissive hit; mesh.slang:1066-1131) in
+ * reflection_colour; any other value = no reflection ray (reflected = 0; reflection_colour and
+ * reflection_base are not read). One call composites many frames in one launch. */
+typedef struct svgf_composite svgf_composite;
+
+typedef struct svgf_composite_frame {
+    uint32_t width, height;          /* render target = history size (camera.screen_width/height) */
+    uint32_t reserved[2];
+    float prev_view_projection[16];  /* row-major, element (r, c) at 4 r + c: clip = M (x, y, z, 1)
+                                        (CameraData.prev_view_projection, mesh.slang:29) */
+    svgf_plane history;              /* previous frame's denoised output: RGBA16F or RGBA32F */
+    svgf_plane emissive;             /* rgb used: RGBA16F or RGBA32F */
+    svgf_plane fresnel;              /* rgb used (F_view): RGBA16F or RGBA32F */
+    svgf_plane reflection;           /* RGBA32F: hit position xyz, kind in w */
+    svgf_plane reflection_colour;    /* rgb used: RGBA16F or RGBA32F */
+    svgf_plane reflection_base;      /* rgb used: RGBA16F or RGBA32F */
+    svgf_plane out;                  /* RGBA16F or RGBA32F; may not alias history */
+} svgf_composite_frame;
+
+typedef struct svgf_composite_params {
+    float ambient; /* scale of the hit base colour when the lookup misses; AMBIENT = 0.03
+                      (mesh.slang:276) */
+    uint32_t reserved[3];
+} svgf_composite_params;
+
+SVGF_API svgf_status svgf_composite_default_params(svgf_composite_params* out);
+SVGF_API svgf_status svgf_composite_create(int32_t device, uint32_t max_frames, svgf_composite** out);
+SVGF_API svgf_status svgf_composite_destroy(svgf_composite* comp);
+/* Enqueues the composite of n frames on hip_stream (asynchronous). Each frame's planes may have
+ * their own formats and pitches; every plane stays below 2 GiB. */
+SVGF_API svgf_status svgf_composite_run(svgf_composite* comp, const svgf_composite_frame* frames, uint32_t n,
+                                        const svgf_composite_params* params, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

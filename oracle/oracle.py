@@ -87,6 +87,11 @@ def lib():
         L.oracle_wire_depth.restype = ctypes.c_float
         L.oracle_wire_encode.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                          fp, fp, ctypes.POINTER(ctypes.c_uint8), fp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_composite_lookup.argtypes = [fp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                              ctypes.c_float, ip, ip]
+        L.oracle_composite.argtypes = [ctypes.c_int, ctypes.c_int, fp, ctypes.c_float, fp, fp, fp, fp, fp, fp,
+                                       ctypes.c_int, fp]
         _lib = L
     return _lib
 
@@ -217,3 +222,34 @@ def wire_encode(colour, gbuffer, exposure: float = 1.0, near: float = WIRE_NEAR,
     if rc:
         raise ValueError(f"oracle_wire_encode failed: {rc}")
     return rgb, depth
+
+
+COMPOSITE_AMBIENT = 0.03  # src/mesh.slang:276
+KIND_NONE, KIND_READY, KIND_REPROJECT = 0.0, 1.0, 2.0
+
+
+def composite_lookup(m, w: int, h: int, pos):
+    """History pixel (x, y) a world position reprojects to through the row-major 4x4 m, or None
+    (behind the camera or off screen) — mesh.slang:1162-1170."""
+    m = _f32(m).reshape(16)
+    px, py = ctypes.c_int(0), ctypes.c_int(0)
+    rc = lib().oracle_composite_lookup(_fp(m), int(w), int(h), float(pos[0]), float(pos[1]), float(pos[2]),
+                                       ctypes.byref(px), ctypes.byref(py))
+    return None if rc else (px.value, py.value)
+
+
+def composite(m, history, emissive, fresnel, reflection, refl_colour, refl_base, ambient: float = COMPOSITE_AMBIENT,
+              store=STORE_F32):
+    """Every plane (h, w, 4) float32 -> composited colour (h, w, 4) float32 (mesh.slang:1007,
+    1161-1174, 1191); store=STORE_F16 rounds the output to half."""
+    planes = [_f32(a) for a in (history, emissive, fresnel, reflection, refl_colour, refl_base)]
+    h, w = planes[0].shape[:2]
+    for a in planes:
+        if a.shape != (h, w, 4):
+            raise ValueError("every plane is (h, w, 4)")
+    out = np.zeros((h, w, 4), np.float32)
+    m = _f32(m).reshape(16)
+    rc = lib().oracle_composite(w, h, _fp(m), float(ambient), *[_fp(a) for a in planes], int(store), _fp(out))
+    if rc:
+        raise ValueError(f"oracle_composite failed: {rc}")
+    return out

@@ -91,6 +91,16 @@ float oracle_wire_depth(float z, float near_z, float far_z);
 int oracle_wire_encode(int w, int h, float exposure, float near_z, float far_z, const float* colour,
                        const float* gbuffer, uint8_t* rgb, float* depth);
 
+/* Consumer of the denoised output (composite_oracle.c): mesh.slang:1007, 1161-1174, 1191.
+ * m: row-major 4x4 previous view-projection (clip = m (x, y, z, 1)). Lookup: the history pixel
+ * a world position reprojects to; 0 and *px, *py set, or -1 when behind the camera / off
+ * screen. Composite: every plane w*h*4 float; reflection.w is the kind (2 reproject,
+ * 1 ready colour, else none); store = 1 rounds the output to half. Returns 0 on success. */
+int oracle_composite_lookup(const float* m, int w, int h, float x, float y, float z, int* px, int* py);
+int oracle_composite(int w, int h, const float* m, float ambient, const float* history, const float* emissive,
+                     const float* fresnel, const float* reflection, const float* refl_colour, const float* refl_base,
+                     int store, float* out);
+
 #ifdef __cplusplus
 }
 #endif

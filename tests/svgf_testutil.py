@@ -92,3 +92,9 @@ def crop_interior(y0, x0, c, w, h):
     bottom = c if y0 + c == h else c - MARGIN
     right = c if x0 + c == w else c - MARGIN
     return slice(top, bottom), slice(left, right)
+
+
+# ---------------------------------------------------------------------------------------------
+# consumer composite scenes (the generator lives with the library's other synthetic inputs)
+# ---------------------------------------------------------------------------------------------
+from svgf_amd.scenes import composite_scene, unproject, view_projection  # noqa: E402,F401

@@ -213,3 +213,25 @@ def test_wire_and_readback_argument_checks_without_gpu():
     if not gpu:
         assert L.svgf_wire_create(0, 16, ctypes.byref(h)) == 8
         assert L.svgf_readback_create(0, 64, 3, ctypes.byref(h)) == 8
+
+
+def test_composite_argument_checks_without_gpu():
+    """Host-side checks of the consumer composite (no kernel launched)."""
+    L = S.lib()
+    p = S.composite_default_params()
+    assert abs(p.ambient - 0.03) < 1e-9  # AMBIENT, mesh.slang:276
+    assert L.svgf_composite_default_params(None) == 1
+    assert ctypes.sizeof(S.CompositeFrame) == 8 + 8 + 64 + 7 * ctypes.sizeof(S.Plane)
+    h = ctypes.c_void_p()
+    assert L.svgf_composite_create(0, 0, ctypes.byref(h)) == 1
+    assert L.svgf_composite_create(0, 1 << 20, ctypes.byref(h)) == 1
+    assert L.svgf_composite_run(None, None, 0, ctypes.byref(p), None) == 1
+    assert L.svgf_composite_destroy(None) == 1
+    try:
+        import torch
+
+        gpu = torch.cuda.is_available()
+    except ImportError:
+        gpu = False
+    if not gpu:
+        assert L.svgf_composite_create(0, 16, ctypes.byref(h)) == 8

@@ -83,6 +83,87 @@ hipError_t launch_batch_persistent(int canonical, const BatchTable& t, const KPa
 // Resident workgroups per CU of the persistent kernel (0 if the query failed).
 int persistent_blocks_per_cu(int canonical, Storage st, bool compensated);
 
+// Makes `dev` current for the scope of the object and restores the caller's device after.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+    DeviceGuard(const DeviceGuard&) = delete;
+    DeviceGuard& operator=(const DeviceGuard&) = delete;
+};
+
+inline bool aligned_to(const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+
+// Per-call descriptor tables of the one-launch-per-call kernels (wire encoder, composite):
+// RING slots of pinned host staging and a device copy. A call fills the next slot's staging,
+// copies it to the device table on the caller's stream and records the slot's event; a slot is
+// written again RING calls later, after its event (the copy that read the staging) completed.
+struct TableRing {
+    static constexpr int RING = 3;
+    struct Slot {
+        void* staging = nullptr;  // pinned
+        void* table = nullptr;    // device
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    Slot slots[RING];
+    size_t bytes = 0;
+    int next = 0;
+
+    bool create(size_t table_bytes)
+    {
+        bytes = table_bytes;
+        for (auto& s : slots) {
+            if (hipHostMalloc(&s.staging, bytes, hipHostMallocDefault) != hipSuccess ||
+                hipMalloc(&s.table, bytes) != hipSuccess ||
+                hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess)
+                return false;
+        }
+        return true;
+    }
+    void destroy()
+    {
+        for (auto& s : slots) {
+            if (s.pending) (void)hipEventSynchronize(s.done);
+            if (s.done) (void)hipEventDestroy(s.done);
+            if (s.staging) (void)hipHostFree(s.staging);
+            if (s.table) (void)hipFree(s.table);
+            s = Slot{};
+        }
+    }
+    // The next slot, with its staging free to write (nullptr on a HIP error).
+    Slot* acquire()
+    {
+        Slot& s = slots[next];
+        next = (next + 1) % RING;
+        if (s.pending) {
+            if (hipEventSynchronize(s.done) != hipSuccess) return nullptr;
+            s.pending = false;
+        }
+        return &s;
+    }
+    // Enqueue the copy of the first `used` bytes of the slot's staging into its device table.
+    static bool upload(Slot& s, size_t used, hipStream_t stream)
+    {
+        return hipMemcpyAsync(s.table, s.staging, used, hipMemcpyHostToDevice, stream) == hipSuccess;
+    }
+    // After the launch that reads the table: the slot is free once this point of the stream is.
+    static bool release(Slot& s, hipStream_t stream)
+    {
+        if (hipEventRecord(s.done, stream) != hipSuccess) return false;
+        s.pending = true;
+        return true;
+    }
+};
+
 }  // namespace svgf
 
 // Read-only accessors of an svgf_context for other parts of the library (server).

@@ -33,7 +33,6 @@ constexpr int WIRE_BLOCK = 256;
 constexpr int WIRE_PX_PER_THREAD = 4;
 constexpr int WIRE_PX_PER_BLOCK = WIRE_BLOCK * WIRE_PX_PER_THREAD;  // 1024
 constexpr uint32_t WIRE_MAX_FRAMES = 4096;
-constexpr int WIRE_SLOTS = 3;
 constexpr int BUF_WORD3 = 0x00020000;
 
 struct WireDesc {
@@ -208,41 +207,17 @@ void make_thresholds(float* t)
     }
 }
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard()
-    {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
-
-bool aligned(const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
-
 }  // namespace
 }  // namespace svgf
 
 using namespace svgf;
-
-struct WireSlot {
-    void* staging = nullptr;  // pinned: starts (max + 1) then descriptors
-    void* table = nullptr;
-    hipEvent_t done = nullptr;
-    bool pending = false;
-};
 
 struct svgf_wire {
     int device = 0;
     uint32_t max_frames = 0;
     size_t frames_off = 0, table_bytes = 0;
     float* thresholds = nullptr;  // device, 255
-    WireSlot slots[WIRE_SLOTS];
-    int next = 0;
+    TableRing ring;                // per call: starts (n + 1), then the descriptors
 };
 
 struct svgf_readback {
@@ -260,12 +235,7 @@ namespace {
 
 void free_wire(svgf_wire* w)
 {
-    for (auto& s : w->slots) {
-        if (s.pending) (void)hipEventSynchronize(s.done);
-        if (s.done) (void)hipEventDestroy(s.done);
-        if (s.staging) (void)hipHostFree(s.staging);
-        if (s.table) (void)hipFree(s.table);
-    }
+    w->ring.destroy();
     if (w->thresholds) (void)hipFree(w->thresholds);
     delete w;
 }
@@ -317,11 +287,7 @@ svgf_status svgf_wire_create(int32_t device, uint32_t max_frames, svgf_wire** ou
     make_thresholds(t);
     bool ok = hipMalloc(reinterpret_cast<void**>(&w->thresholds), sizeof(t)) == hipSuccess &&
               hipMemcpy(w->thresholds, t, sizeof(t), hipMemcpyHostToDevice) == hipSuccess;
-    for (auto& s : w->slots) {
-        ok = ok && hipHostMalloc(&s.staging, w->table_bytes, hipHostMallocDefault) == hipSuccess &&
-             hipMalloc(&s.table, w->table_bytes) == hipSuccess &&
-             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) == hipSuccess;
-    }
+    ok = ok && w->ring.create(w->table_bytes);
     if (!ok) {
         free_wire(w);
         return SVGF_ERR_OUT_OF_MEMORY;
@@ -357,8 +323,8 @@ svgf_status svgf_wire_encode(svgf_wire* w, const svgf_wire_frame* frames, uint32
         fp16 = h16;
         const uint64_t cb = h16 ? 8 : 16;
         if (f.gbuffer.format != SVGF_FMT_RGBA32F || f.colour.pitch_bytes < f.width * cb || f.colour.pitch_bytes % cb ||
-            f.gbuffer.pitch_bytes < (uint64_t)f.width * 16 || f.gbuffer.pitch_bytes % 16 || !aligned(f.colour.ptr, cb) ||
-            !aligned(f.gbuffer.ptr, 16) || !aligned(f.rgb, 4) || !aligned(f.depth, 16))
+            f.gbuffer.pitch_bytes < (uint64_t)f.width * 16 || f.gbuffer.pitch_bytes % 16 || !aligned_to(f.colour.ptr, cb) ||
+            !aligned_to(f.gbuffer.ptr, 16) || !aligned_to(f.rgb, 4) || !aligned_to(f.depth, 16))
             return SVGF_ERR_INVALID_ARGUMENT;
         const uint64_t px = (uint64_t)f.width * f.height;
         if (f.colour.pitch_bytes * f.height >= (1ull << 31) || f.gbuffer.pitch_bytes * f.height >= (1ull << 31) ||
@@ -368,12 +334,9 @@ svgf_status svgf_wire_encode(svgf_wire* w, const svgf_wire_frame* frames, uint32
     }
     if (blocks >= (1ull << 31)) return SVGF_ERR_CAPACITY;
     DeviceGuard guard(w->device);
-    WireSlot& s = w->slots[w->next];
-    w->next = (w->next + 1) % WIRE_SLOTS;
-    if (s.pending) {
-        if (hipEventSynchronize(s.done) != hipSuccess) return SVGF_ERR_HIP;
-        s.pending = false;
-    }
+    TableRing::Slot* slot = w->ring.acquire();
+    if (!slot) return SVGF_ERR_HIP;
+    TableRing::Slot& s = *slot;
     uint32_t* starts = static_cast<uint32_t*>(s.staging);
     WireDesc* descs = reinterpret_cast<WireDesc*>(static_cast<char*>(s.staging) + w->frames_off);
     uint32_t acc = 0;
@@ -396,7 +359,7 @@ svgf_status svgf_wire_encode(svgf_wire* w, const svgf_wire_frame* frames, uint32
     starts[n] = acc;
     hipStream_t stream = static_cast<hipStream_t>(hip_stream);
     const size_t copy = w->frames_off + sizeof(WireDesc) * n;
-    if (hipMemcpyAsync(s.table, s.staging, copy, hipMemcpyHostToDevice, stream) != hipSuccess) return SVGF_ERR_HIP;
+    if (!TableRing::upload(s, copy, stream)) return SVGF_ERR_HIP;
     WireArgs a{};
     a.starts = static_cast<const uint32_t*>(s.table);
     a.frames = reinterpret_cast<const WireDesc*>(static_cast<const char*>(s.table) + w->frames_off);
@@ -408,8 +371,7 @@ svgf_status svgf_wire_encode(svgf_wire* w, const svgf_wire_frame* frames, uint32
     if (fp16) hipLaunchKernelGGL(k_wire<true>, dim3(acc), dim3(WIRE_BLOCK), 0, stream, a);
     else hipLaunchKernelGGL(k_wire<false>, dim3(acc), dim3(WIRE_BLOCK), 0, stream, a);
     if (hipGetLastError() != hipSuccess) return SVGF_ERR_HIP;
-    if (hipEventRecord(s.done, stream) != hipSuccess) return SVGF_ERR_HIP;
-    s.pending = true;
+    if (!TableRing::release(s, stream)) return SVGF_ERR_HIP;
     return SVGF_OK;
 }
 

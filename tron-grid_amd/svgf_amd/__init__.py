@@ -109,6 +109,18 @@ class WireFrame(ctypes.Structure):
                 ("rgb", ctypes.c_void_p), ("depth", ctypes.c_void_p)]
 
 
+COMPOSITE_PLANES = ("history", "emissive", "fresnel", "reflection", "reflection_colour", "reflection_base", "out")
+
+
+class CompositeFrame(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2),
+                ("prev_view_projection", ctypes.c_float * 16)] + [(n, Plane) for n in COMPOSITE_PLANES]
+
+
+class CompositeParams(ctypes.Structure):
+    _fields_ = [("ambient", ctypes.c_float), ("reserved", ctypes.c_uint32 * 3)]
+
+
 class StreamState(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("frames", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("history", Plane), ("moments", Plane), ("gbuffer", Plane)]
@@ -213,6 +225,10 @@ def lib():
             "svgf_readback_destroy": [vp],
             "svgf_readback_push": [vp, vp, ctypes.c_uint64, vp, P(ctypes.c_uint64)],
             "svgf_readback_view": [vp, ctypes.c_uint64, ctypes.c_int32, P(vp), P(ctypes.c_uint64)],
+            "svgf_composite_default_params": [P(CompositeParams)],
+            "svgf_composite_create": [ctypes.c_int32, ctypes.c_uint32, P(vp)],
+            "svgf_composite_destroy": [vp],
+            "svgf_composite_run": [vp, P(CompositeFrame), ctypes.c_uint32, P(CompositeParams), vp],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args
@@ -571,3 +587,63 @@ class Readback:
         check(lib().svgf_readback_view(self.handle, seq, 1 if wait else 0, ctypes.byref(ptr), ctypes.byref(n)),
               "svgf_readback_view")
         return np.ctypeslib.as_array((ctypes.c_uint8 * n.value).from_address(ptr.value))
+
+
+# ---------------------------------------------------------------------------------------------
+# consumer composite (one-frame-late history + reprojected reflections)
+# ---------------------------------------------------------------------------------------------
+KIND_NONE, KIND_READY, KIND_REPROJECT = 0.0, 1.0, 2.0
+
+
+def composite_default_params() -> CompositeParams:
+    p = CompositeParams()
+    check(lib().svgf_composite_default_params(ctypes.byref(p)), "svgf_composite_default_params")
+    return p
+
+
+class Composite:
+    """Batched consumer composite (svgf_composite_*): out = (history + emissive) + reflected * F_view."""
+
+    def __init__(self, max_frames=256, device=0):
+        self.handle = ctypes.c_void_p()
+        check(lib().svgf_composite_create(device, max_frames, ctypes.byref(self.handle)), "svgf_composite_create")
+
+    def close(self):
+        if self.handle:
+            check(lib().svgf_composite_destroy(self.handle), "svgf_composite_destroy")
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def frames(items):
+        """ctypes array of svgf_composite_frame from dicts holding every plane of COMPOSITE_PLANES as a
+        (h, w, 4) fp16/fp32 device tensor (reflection fp32) and prev_view_projection (16 floats,
+        row-major)."""
+        torch = _torch()
+        arr = (CompositeFrame * len(items))()
+        for k, it in enumerate(items):
+            h, w = it["history"].shape[:2]
+            f = arr[k]
+            f.width, f.height = w, h
+            f.prev_view_projection[:] = [float(v) for v in np.asarray(it["prev_view_projection"], np.float32).reshape(16)]
+            for name in COMPOSITE_PLANES:
+                t = it[name]
+                setattr(f, name, plane_of(t, FMT_RGBA16F if t.dtype == torch.float16 else FMT_RGBA32F))
+        return arr
+
+    def run(self, items, params=None, stream=None, frames=None):
+        arr = frames if frames is not None else self.frames(items)
+        p = params or composite_default_params()
+        check(lib().svgf_composite_run(self.handle, arr, len(arr), ctypes.byref(p), _stream_handle(stream)),
+              "svgf_composite_run")
