@@ -356,6 +356,87 @@ def composite_report(reps: int, cpu: bool) -> dict:
     return out
 
 
+# Producer pack (src/main.cpp:595-646, src/mesh.slang:995-998), algorithmic bytes per pixel:
+# every pixel reads its position (RGBA32F) and octahedral normal (R32UI) and writes radiance,
+# G-buffer (RGBA32F) and motion (RG32F); a covered pixel also reads its three lighting planes.
+def pack_bytes(scene, light_bytes):
+    cov = int(scene["covered"].sum())
+    npx = scene["covered"].size
+    return int(npx * (16 + 4 + light_bytes + 16 + 8) + cov * 3 * light_bytes)
+
+
+def pack_report(reps: int, cpu: bool) -> dict:
+    """The producer pack as one launch: lighting terms, world position and octahedral normal ->
+    radiance (zero where no opaque geometry), G-buffer (normal, linear depth) and motion, for
+    1080p and 4K frames (FP16 and FP32 lighting / radiance) and the 64-frame mix in one call.
+    Scene: top quarter of the rows sky, 10 % of the rest transparent-only (both uncovered), a
+    camera that moved 0.02 m since the previous frame."""
+    import numpy as np
+    import torch
+
+    import svgf_amd as S
+    from svgf_amd.scenes import pack_scene
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    L = S.lib()
+    rng = np.random.default_rng(0)
+    mix = [RESOLUTIONS[i] for i in rng.integers(0, len(RESOLUTIONS), 64)]
+    cases = [("1920x1080_fp16", [(1920, 1080)], True), ("3840x2160_fp16", [(3840, 2160)], True),
+             ("3840x2160_fp32", [(3840, 2160)], False), ("mix64_fp16", mix, True)]
+    out = {"what": " ".join(ln.strip() for ln in pack_report.__doc__.split("\n")).strip()}
+    pk = S.Pack(max_frames=64)
+    for name, sizes, half in cases:
+        scenes = [pack_scene(w, h, 700 + k, half=half) for k, (w, h) in enumerate(sizes)]
+        items = []
+        for d in scenes:
+            h, w = d["position"].shape[:2]
+            it = {"view_projection": d["vp"], "prev_view_projection": d["pvp"],
+                  "position": torch.from_numpy(d["position"]).to(dev),
+                  "normal_oct": torch.from_numpy(d["normal_oct"].view(np.int32)).to(dev),
+                  "radiance": torch.empty((h, w, 4), dtype=torch.float16 if half else torch.float32, device=dev),
+                  "gbuffer": torch.empty((h, w, 4), dtype=torch.float32, device=dev),
+                  "motion": torch.empty((h, w, 2), dtype=torch.float32, device=dev)}
+            for k in ("indirect", "direct_diffuse", "direct_specular"):
+                t = torch.from_numpy(d[k]).to(dev)
+                it[k] = t.half() if half else t
+            items.append(it)
+        arr = S.Pack.frames(items)
+        nbytes = sum(pack_bytes(d, 8 if half else 16) for d in scenes)
+        npx = sum(w * h for w, h in sizes)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            for _ in range(10):
+                S.check(L.svgf_pack_run(pk.handle, arr, len(arr), sh), "svgf_pack_run")
+            e0.record(stream)
+            for _ in range(reps):
+                S.check(L.svgf_pack_run(pk.handle, arr, len(arr), sh), "svgf_pack_run")
+            e1.record(stream)
+            e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbps = nbytes / ms / 1e6
+        out[name] = {"frames": len(sizes), "pixels": npx, "ms": round(ms, 5), "mpx_per_s": round(npx / ms / 1e3, 1),
+                     "algorithmic_bytes": nbytes, "bytes_per_px": round(nbytes / npx, 2),
+                     "achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+        if cpu and name == "1920x1080_fp16":
+            sys.path.insert(0, str(ROOT / "oracle"))
+            import oracle as O
+
+            d = scenes[0]
+            r, t0 = 0, time.perf_counter()
+            while r < 3 or time.perf_counter() - t0 < 1.0:
+                O.pack(d["vp"], d["pvp"], d["position"], d["normal_oct"], d["indirect"], d["direct_diffuse"],
+                       d["direct_specular"], store=O.STORE_F16)
+                r += 1
+            out["cpu_1920x1080"] = {"mpx_per_s": round(1920 * 1080 * r / (time.perf_counter() - t0) / 1e6, 2),
+                                    "cores": 1, "kind": "port", "what": "oracle (C, 1 thread)"}
+        del items, arr
+    pk.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -367,7 +448,7 @@ def main():
     ap.add_argument("--agents", type=int, default=256, help="agent streams of the agent-frame tick (0: skip)")
     ap.add_argument("--agent-ticks", type=int, default=20)
     ap.add_argument("--composite-reps", type=int, default=50,
-                    help="launches timed per consumer-composite case (0: skip)")
+                    help="launches timed per consumer-composite and producer-pack case (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="serving simulation length (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4000.0, help="mean Poisson arrival rate, requests/s")
@@ -376,9 +457,10 @@ def main():
                     help="HBM bytes per a-trous launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE (reported as is)")
     ap.add_argument("--no-ref720", dest="ref720", action="store_false",
                     help="skip the REFERENCE-mode 1280x720 timing (the reference's published case)")
-    ap.add_argument("--report", choices=["precision", "copy", "composite"], default=None,
+    ap.add_argument("--report", choices=["precision", "copy", "composite", "pack"], default=None,
                     help="precision: error of the GPU forms vs the FP64 oracle on 16-frame 1080p / 4K sequences; "
-                         "copy: one 1 GiB device copy (counter calibration); composite: the consumer composite only")
+                         "copy: one 1 GiB device copy (counter calibration); composite / pack: the consumer composite / "
+                         "the producer pack only")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched form (for profiler runs)")
     ap.add_argument("--launch", choices=["persistent", "per_stage"], default="per_stage",
@@ -389,6 +471,9 @@ def main():
     ap.add_argument("--report-threads", type=int, default=16)
     args = ap.parse_args()
 
+    if args.report == "pack":
+        print(json.dumps({"producer_pack": pack_report(args.composite_reps, not args.no_cpu_baseline)}))
+        return
     if args.report == "composite":
         print(json.dumps({"consumer_composite": composite_report(args.composite_reps, not args.no_cpu_baseline)}))
         return
@@ -631,6 +716,7 @@ def main():
 
     # ---- consumer composite: the reader of the denoised output ----
     composite = composite_report(args.composite_reps, not args.no_cpu_baseline) if args.composite_reps > 0 else None
+    pack = pack_report(args.composite_reps, not args.no_cpu_baseline) if args.composite_reps > 0 else None
 
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
     cpu = None
@@ -720,6 +806,7 @@ def main():
             "streams_opened": int(serving["streams_opened"])},
         "agent_frames": agent,
         "consumer_composite": composite,
+        "producer_pack": pack,
         "batched_other_launch": other,
         "plain_fp32": {"value": round(plain_mpx, 2), "unit": "Mpx/s", "bytes_per_px": bytes_px_32,
                        "ratio_batched_over_plain": round(batched_mpx / plain_mpx, 3)},

@@ -69,7 +69,8 @@ typedef enum svgf_format {
     SVGF_FMT_RGBA32F = 1, /* 16 B/px */
     SVGF_FMT_RGBA16F = 2, /*  8 B/px */
     SVGF_FMT_RG32F = 3,   /*  8 B/px */
-    SVGF_FMT_RG16F = 4    /*  4 B/px */
+    SVGF_FMT_RG16F = 4,   /*  4 B/px */
+    SVGF_FMT_R32UI = 5    /*  4 B/px: packed codes (octahedral normals of svgf_pack) */
 } svgf_format;
 
 /* Flags of svgf_params.flags */
@@ -405,6 +406,41 @@ SVGF_API svgf_status svgf_composite_destroy(svgf_composite* comp);
  * their own formats and pitches; every plane stays below 2 GiB. */
 SVGF_API svgf_status svgf_composite_run(svgf_composite* comp, const svgf_composite_frame* frames, uint32_t n,
                                         const svgf_composite_params* params, void* hip_stream);
+
+/* --- producer pack --------------------------------------------------------------------------
+ * Builds the denoiser's input planes from what TronGrid's mesh pass holds per pixel, writing
+ * every pixel once (the per-frame zero-clear of src/main.cpp:595-646 folded into the pass):
+ *   covered (position.w > 0):
+ *     radiance = ((indirect + direct_diffuse) * ao + direct_specular, 1)  (mesh.slang:995-998)
+ *     gbuffer  = (unpackOctahedralNormal(normal_oct), (VP (p, 1)).w)      (mesh.slang:341-352)
+ *     motion   = previous position - pixel centre, in pixels, previous position from
+ *                clip = PVP (p, 1): (clip.xy / max(clip.w, 1e-6) (0.5, -0.5) + 0.5) (W, H)
+ *                (mesh.slang:589-591); (-1e6, -1e6) when clip.w <= 0 or |motion| > 1e6 px
+ *   not covered (sky, transparent-only): radiance = 0, gbuffer = 0, motion = (-1e6, -1e6).
+ * The depth (VP (p, 1)).w is the view-space distance along the camera axis of a perspective
+ * projection. Normals are normalised with a correctly rounded square root and division.
+ * Arithmetic is float, one rounding per operation, matrix rows summed left to right. */
+typedef struct svgf_pack svgf_pack;
+
+typedef struct svgf_pack_frame {
+    uint32_t width, height;
+    uint32_t reserved[2];
+    float view_projection[16];      /* row-major, this frame (CameraData.view x projection) */
+    float prev_view_projection[16]; /* row-major, previous frame (mesh.slang:29) */
+    svgf_plane position;            /* RGBA32F: world position xyz, coverage in w */
+    svgf_plane normal_oct;          /* R32UI: octahedral code (mesh.slang:327-339) */
+    svgf_plane indirect;            /* rgb used: RGBA16F or RGBA32F */
+    svgf_plane direct_diffuse;      /* rgb + ao_factor in w: RGBA16F or RGBA32F */
+    svgf_plane direct_specular;     /* rgb used: RGBA16F or RGBA32F */
+    svgf_plane radiance;            /* out: RGBA16F or RGBA32F (the denoiser's radiance plane) */
+    svgf_plane gbuffer;             /* out: RGBA32F normal.xyz + linear depth */
+    svgf_plane motion;              /* out: RG32F, pixels */
+} svgf_pack_frame;
+
+SVGF_API svgf_status svgf_pack_create(int32_t device, uint32_t max_frames, svgf_pack** out);
+SVGF_API svgf_status svgf_pack_destroy(svgf_pack* pack);
+/* Enqueues the pack of n frames on hip_stream (asynchronous). Outputs may not overlap inputs. */
+SVGF_API svgf_status svgf_pack_run(svgf_pack* pack, const svgf_pack_frame* frames, uint32_t n, void* hip_stream);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,9 @@ def lib():
                                               ctypes.c_float, ip, ip]
         L.oracle_composite.argtypes = [ctypes.c_int, ctypes.c_int, fp, ctypes.c_float, fp, fp, fp, fp, fp, fp,
                                        ctypes.c_int, fp]
+        up = ctypes.POINTER(ctypes.c_uint32)
+        L.oracle_unpack_octahedral.argtypes = [ctypes.c_uint32, fp]
+        L.oracle_pack.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, fp, up, fp, fp, fp, ctypes.c_int, fp, fp, fp]
         _lib = L
     return _lib
 
@@ -253,3 +256,36 @@ def composite(m, history, emissive, fresnel, reflection, refl_colour, refl_base,
     if rc:
         raise ValueError(f"oracle_composite failed: {rc}")
     return out
+
+
+PACK_NO_MOTION = -1.0e6  # motion of a pixel with no previous position
+
+
+def unpack_octahedral(packed: int) -> np.ndarray:
+    """The unit normal of a 16 + 16-bit octahedral code (mesh.slang:341-352)."""
+    n = np.zeros(3, np.float32)
+    lib().oracle_unpack_octahedral(int(packed) & 0xFFFFFFFF, _fp(n))
+    return n
+
+
+def pack(vp, pvp, position, normal_oct, indirect, direct_diffuse, direct_specular, store=STORE_F32):
+    """Producer pack (main.cpp:595-646, mesh.slang:995-998): returns (radiance (h, w, 4),
+    gbuffer (h, w, 4), motion (h, w, 2)) float32."""
+    position, indirect, direct_diffuse, direct_specular = (_f32(a) for a in (position, indirect, direct_diffuse,
+                                                                              direct_specular))
+    normal_oct = np.ascontiguousarray(normal_oct, dtype=np.uint32)
+    h, w = position.shape[:2]
+    for a in (indirect, direct_diffuse, direct_specular):
+        if a.shape != (h, w, 4):
+            raise ValueError("lighting planes are (h, w, 4)")
+    if normal_oct.shape != (h, w):
+        raise ValueError("normal_oct is (h, w)")
+    rad = np.zeros((h, w, 4), np.float32)
+    g = np.zeros((h, w, 4), np.float32)
+    mv = np.zeros((h, w, 2), np.float32)
+    rc = lib().oracle_pack(w, h, _fp(_f32(vp).reshape(16)), _fp(_f32(pvp).reshape(16)), _fp(position),
+                           normal_oct.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _fp(indirect),
+                           _fp(direct_diffuse), _fp(direct_specular), int(store), _fp(rad), _fp(g), _fp(mv))
+    if rc:
+        raise ValueError(f"oracle_pack failed: {rc}")
+    return rad, g, mv

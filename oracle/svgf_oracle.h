@@ -101,6 +101,16 @@ int oracle_composite(int w, int h, const float* m, float ambient, const float* h
                      const float* fresnel, const float* reflection, const float* refl_colour, const float* refl_base,
                      int store, float* out);
 
+/* Producer side of the denoiser input (pack_oracle.c): main.cpp:595-646 (zero-clear),
+ * mesh.slang:995-998 (lighting), :341-352 (octahedral normal), :589-591 (previous position).
+ * vp, pvp: row-major 4x4 current / previous view-projection. position, indirect,
+ * direct_diffuse (ao in .w), direct_specular: w*h*4 float; normal_oct: w*h uint32. Outputs:
+ * radiance w*h*4 (store = 1 rounds to half), gbuffer w*h*4 (normal, depth), motion w*h*2. */
+void oracle_unpack_octahedral(uint32_t packed, float* n3);
+int oracle_pack(int w, int h, const float* vp, const float* pvp, const float* position, const uint32_t* normal_oct,
+                const float* indirect, const float* direct_diffuse, const float* direct_specular, int store,
+                float* radiance, float* gbuffer, float* motion);
+
 #ifdef __cplusplus
 }
 #endif

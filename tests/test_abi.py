@@ -235,3 +235,22 @@ def test_composite_argument_checks_without_gpu():
         gpu = False
     if not gpu:
         assert L.svgf_composite_create(0, 16, ctypes.byref(h)) == 8
+
+
+def test_pack_argument_checks_without_gpu():
+    """Host-side checks of the producer pack (no kernel launched)."""
+    L = S.lib()
+    assert ctypes.sizeof(S.PackFrame) == 16 + 128 + 8 * ctypes.sizeof(S.Plane)
+    h = ctypes.c_void_p()
+    assert L.svgf_pack_create(0, 0, ctypes.byref(h)) == 1
+    assert L.svgf_pack_create(0, 1 << 20, ctypes.byref(h)) == 1
+    assert L.svgf_pack_run(None, None, 0, None) == 1
+    assert L.svgf_pack_destroy(None) == 1
+    try:
+        import torch
+
+        gpu = torch.cuda.is_available()
+    except ImportError:
+        gpu = False
+    if not gpu:
+        assert L.svgf_pack_create(0, 16, ctypes.byref(h)) == 8

@@ -27,7 +27,7 @@ MODE_REFERENCE = 0
 MODE_CANONICAL = 1
 STORAGE_FP32 = 0
 STORAGE_FP16 = 1
-FMT_RGBA32F, FMT_RGBA16F, FMT_RG32F, FMT_RG16F = 1, 2, 3, 4
+FMT_RGBA32F, FMT_RGBA16F, FMT_RG32F, FMT_RG16F, FMT_R32UI = 1, 2, 3, 4, 5
 FLAG_COMPENSATED = 0x1
 BATCH_NO_GRAPH = 0x1
 PROFILE_STAGES = 0x2
@@ -115,6 +115,16 @@ COMPOSITE_PLANES = ("history", "emissive", "fresnel", "reflection", "reflection_
 class CompositeFrame(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2),
                 ("prev_view_projection", ctypes.c_float * 16)] + [(n, Plane) for n in COMPOSITE_PLANES]
+
+
+PACK_INPUTS = ("position", "normal_oct", "indirect", "direct_diffuse", "direct_specular")
+PACK_OUTPUTS = ("radiance", "gbuffer", "motion")
+
+
+class PackFrame(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2),
+                ("view_projection", ctypes.c_float * 16), ("prev_view_projection", ctypes.c_float * 16)] + [
+                    (n, Plane) for n in PACK_INPUTS + PACK_OUTPUTS]
 
 
 class CompositeParams(ctypes.Structure):
@@ -229,6 +239,9 @@ def lib():
             "svgf_composite_create": [ctypes.c_int32, ctypes.c_uint32, P(vp)],
             "svgf_composite_destroy": [vp],
             "svgf_composite_run": [vp, P(CompositeFrame), ctypes.c_uint32, P(CompositeParams), vp],
+            "svgf_pack_create": [ctypes.c_int32, ctypes.c_uint32, P(vp)],
+            "svgf_pack_destroy": [vp],
+            "svgf_pack_run": [vp, P(PackFrame), ctypes.c_uint32, vp],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args
@@ -647,3 +660,61 @@ class Composite:
         p = params or composite_default_params()
         check(lib().svgf_composite_run(self.handle, arr, len(arr), ctypes.byref(p), _stream_handle(stream)),
               "svgf_composite_run")
+
+
+# ---------------------------------------------------------------------------------------------
+# producer pack (lighting terms + position + octahedral normal -> radiance, G-buffer, motion)
+# ---------------------------------------------------------------------------------------------
+NO_MOTION = -1.0e6
+
+
+class Pack:
+    """Batched producer pack (svgf_pack_*)."""
+
+    def __init__(self, max_frames=256, device=0):
+        self.handle = ctypes.c_void_p()
+        check(lib().svgf_pack_create(device, max_frames, ctypes.byref(self.handle)), "svgf_pack_create")
+
+    def close(self):
+        if self.handle:
+            check(lib().svgf_pack_destroy(self.handle), "svgf_pack_destroy")
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def frames(items):
+        """ctypes array of svgf_pack_frame from dicts holding view_projection and
+        prev_view_projection (16 floats, row-major), position (h, w, 4) fp32, normal_oct (h, w)
+        int32/uint32, indirect / direct_diffuse / direct_specular (h, w, 4) fp16 or fp32, and the
+        outputs radiance (h, w, 4) fp16/fp32, gbuffer (h, w, 4) fp32, motion (h, w, 2) fp32."""
+        torch = _torch()
+        arr = (PackFrame * len(items))()
+        for k, it in enumerate(items):
+            h, w = it["position"].shape[:2]
+            f = arr[k]
+            f.width, f.height = w, h
+            f.view_projection[:] = [float(v) for v in np.asarray(it["view_projection"], np.float32).reshape(16)]
+            f.prev_view_projection[:] = [float(v) for v in np.asarray(it["prev_view_projection"], np.float32).reshape(16)]
+            f.position = plane_of(it["position"], FMT_RGBA32F)
+            f.normal_oct = plane_of(it["normal_oct"], FMT_R32UI)
+            for name in ("indirect", "direct_diffuse", "direct_specular", "radiance"):
+                t = it[name]
+                setattr(f, name, plane_of(t, FMT_RGBA16F if t.dtype == torch.float16 else FMT_RGBA32F))
+            f.gbuffer = plane_of(it["gbuffer"], FMT_RGBA32F)
+            f.motion = plane_of(it["motion"], FMT_RG32F)
+        return arr
+
+    def run(self, items, stream=None, frames=None):
+        arr = frames if frames is not None else self.frames(items)
+        check(lib().svgf_pack_run(self.handle, arr, len(arr), _stream_handle(stream)), "svgf_pack_run")

@@ -12,14 +12,11 @@ from __future__ import annotations
 import numpy as np
 
 
-def view_projection(w, h, seed):
-    """Row-major float64 4x4 P V of a perspective camera (60 degree vertical field of view, near
-    0.1, far 100, clip z in [0, w], w = -z_view) looking at the origin from a seeded position;
-    returns (M, eye)."""
-    rng = np.random.default_rng(seed)
-    eye = rng.uniform(-6.0, 6.0, 3) + np.array([0.0, 4.0, 0.0])
-    target = rng.uniform(-1.0, 1.0, 3)
-    fwd = target - eye
+def look_at_projection(w, h, eye, target):
+    """Row-major float64 4x4 P V of a perspective camera at `eye` looking at `target` (60 degree
+    vertical field of view, near 0.1, far 100, clip z in [0, w], clip w = -z_view)."""
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(target, np.float64) - eye
     fwd /= np.linalg.norm(fwd)
     side = np.cross(fwd, [0.0, 1.0, 0.0])
     side /= np.linalg.norm(side)
@@ -33,7 +30,15 @@ def view_projection(w, h, seed):
     P[0, 0], P[1, 1] = t * h / w, t
     P[2, 2], P[2, 3] = f / (n - f), f * n / (n - f)
     P[3, 2] = -1.0
-    return P @ V, eye
+    return P @ V
+
+
+def view_projection(w, h, seed):
+    """look_at_projection from a seeded position towards the origin; returns (M, eye)."""
+    rng = np.random.default_rng(seed)
+    eye = rng.uniform(-6.0, 6.0, 3) + np.array([0.0, 4.0, 0.0])
+    target = rng.uniform(-1.0, 1.0, 3)
+    return look_at_projection(w, h, eye, target), eye
 
 
 def unproject(M, ndc_x, ndc_y, z_ndc):
@@ -126,3 +131,59 @@ def composite_scene(w, h, seed, half=False, coherent=False):
     tflat[reproj] = np.where(inside[reproj, None], np.stack([px, py], 1)[reproj], -1)
     return {"m": M32.reshape(16), "history": hist, "emissive": emis, "fresnel": fres, "reflection": refl,
             "reflection_colour": rcol, "reflection_base": rbase, "target": target}
+
+
+def pack_octahedral(n):
+    """The reference's 16 + 16-bit octahedral code of unit vectors n (..., 3)
+    (src/mesh.slang:327-339), evaluated in float64: the producer's encoder."""
+    n = np.asarray(n, np.float64)
+    n = n / np.linalg.norm(n, axis=-1, keepdims=True)
+    s = np.abs(n).sum(-1, keepdims=True)
+    p = n[..., :2] / np.maximum(s, 1e-6)
+    snap = np.where(p >= 0.0, 1.0, -1.0)
+    fold = (1.0 - np.abs(p[..., ::-1])) * snap
+    p = np.where(n[..., 2:3] < 0.0, fold, p)
+    q = np.floor(np.clip(p * 0.5 + 0.5, 0.0, 1.0) * 65535.0 + 0.5).astype(np.uint32)
+    return q[..., 0] | (q[..., 1] << 16)
+
+
+def pack_scene(w, h, seed, half=False, move=0.02):
+    """Seeded producer inputs of one frame: a camera (vp) that moved by `move` metres since the
+    previous frame (pvp), world positions of the pixel centres at random depths with coverage in
+    position.w (the top quarter of the rows is sky, 10 % of the others transparent-only, both
+    uncovered; 2 % of covered pixels sit behind the previous camera), octahedral-coded normals,
+    and the lighting terms indirect, direct_diffuse (ao in .w), direct_specular ((h, w, 4);
+    half-representable when half=True). Also returns depth ((h, w) float64: clip.w of the
+    float32 position through the float32 vp) and normal ((h, w, 3) float64 unit normals)."""
+    rng = np.random.default_rng(seed)
+    eye = rng.uniform(-6.0, 6.0, 3) + np.array([0.0, 4.0, 0.0])
+    target = rng.uniform(-1.0, 1.0, 3)
+    vp = look_at_projection(w, h, eye, target)
+    d = rng.normal(size=3)
+    peye = eye + move * d / np.linalg.norm(d)
+    pvp = look_at_projection(w, h, peye, target + 0.5 * move * rng.normal(size=3))
+    vp32, pvp32 = vp.astype(np.float32), pvp.astype(np.float32)
+    ys, xs = np.mgrid[0:h, 0:w]
+    zn = rng.uniform(0.02, 0.98, (h, w))
+    pos = np.zeros((h, w, 4), np.float32)
+    world = unproject(vp, 2.0 * (xs + 0.5) / w - 1.0, 1.0 - 2.0 * (ys + 0.5) / h, zn)
+    covered = (ys >= h // 4) & (rng.random((h, w)) >= 0.1)
+    behind = covered & (rng.random((h, w)) < 0.02)
+    fwd_prev = target - peye
+    fwd_prev /= np.linalg.norm(fwd_prev)
+    world[behind] = peye - fwd_prev * rng.uniform(0.5, 5.0, (int(behind.sum()), 1))
+    pos[..., :3] = world
+    pos[..., 3] = covered.astype(np.float32)
+    pos[~covered, :3] = rng.uniform(-50.0, 50.0, (int((~covered).sum()), 3))
+    normal = rng.normal(size=(h, w, 3))
+    normal /= np.linalg.norm(normal, axis=-1, keepdims=True)
+    q = (lambda a: a.astype(np.float16).astype(np.float32)) if half else (lambda a: a.astype(np.float32))
+    ind = q(rng.gamma(1.0, 1.0, (h, w, 4)))
+    dd = q(rng.gamma(1.0, 0.5, (h, w, 4)))
+    dd[..., 3] = q(rng.uniform(0.0, 1.0, (h, w)))
+    ds = q(rng.gamma(0.5, 0.2, (h, w, 4)))
+    p64 = np.concatenate([pos[..., :3].astype(np.float64), np.ones((h, w, 1))], -1)
+    depth = p64 @ vp32.astype(np.float64)[3]
+    return {"vp": vp32.reshape(16), "pvp": pvp32.reshape(16), "position": pos, "normal_oct": pack_octahedral(normal),
+            "indirect": ind, "direct_diffuse": dd, "direct_specular": ds, "depth": depth, "normal": normal,
+            "covered": covered, "behind": behind}
