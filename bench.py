@@ -28,6 +28,12 @@ sys.path.insert(0, str(ROOT / "tron-grid_amd"))
 
 RESOLUTIONS = [(96, 96), (128, 128), (256, 256), (320, 180), (640, 480), (1280, 720)]
 HBM_PEAK_GBPS = 8000.0
+_T0 = time.time()
+
+
+def progress(msg: str) -> None:
+    """One line per finished section on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.time() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 # Algorithmic bytes per pixel, per stage, stage-structured rule: each stage reads each distinct
 # plane once and writes each output plane once at its storage width (extra taps count 0).
@@ -312,6 +318,7 @@ def composite_report(reps: int, cpu: bool) -> dict:
         ln.strip() for ln in composite_report.__doc__.split("\n")[1:]).strip()}
     comp = S.Composite(max_frames=64)
     for name, sizes, half, coherent in cases:
+        progress(f"composite {name}")
         scenes = [composite_scene(w, h, 900 + k, half=half, coherent=coherent) for k, (w, h) in enumerate(sizes)]
         items = []
         for d in scenes:
@@ -388,6 +395,7 @@ def pack_report(reps: int, cpu: bool) -> dict:
     out = {"what": " ".join(ln.strip() for ln in pack_report.__doc__.split("\n")).strip()}
     pk = S.Pack(max_frames=64)
     for name, sizes, half in cases:
+        progress(f"pack {name}")
         scenes = [pack_scene(w, h, 700 + k, half=half) for k, (w, h) in enumerate(sizes)]
         items = []
         for d in scenes:
@@ -517,6 +525,7 @@ def main():
     ND = args.distinct
     assert ND % 2 == 0, "distinct frames must be even (ping-pong parity)"
 
+    progress("inputs")
     # ---- inputs: ND frames per stream, generated on the host, resident in HBM ----
     t0 = time.time()
     host = [[None] * 2 for _ in shapes]  # frames 0 and 1 stay on the host for the CPU baseline
@@ -536,6 +545,7 @@ def main():
     torch.cuda.synchronize()
     gen_s = time.time() - t0
 
+    progress("batched FP16 canonical form")
     # ---- batched FP16 canonical form: per-stage launches and one persistent launch per batch ----
     p16 = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
     out16 = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in shapes]
@@ -599,6 +609,7 @@ def main():
                           "ms_per_step": round(t_batched / args.steps * 1e3, 4), "pixels_per_batch": batch_px}))
         return
 
+    progress("plain FP32 canonical form on the same mix")
     # ---- plain FP32 canonical form on the same mix: one frame per launch sequence, no graph ----
     p32 = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP32)
     ctx32 = S.Context(S.MODE_CANONICAL, S.STORAGE_FP32, params=p32, max_width=maxw, max_height=maxh)
@@ -638,6 +649,7 @@ def main():
         t_plain = time.perf_counter() - t0
     plain_mpx = args.steps * batch_px / t_plain / 1e6
 
+    progress("profiled pass")
     # ---- profiled pass: per-stage kernel time (hipEvents around every stage launch) ----
     ctxp = batched_context(S.PROFILE_STAGES)
     stage_ms = []
@@ -659,6 +671,7 @@ def main():
     bytes_px_16 = sum(sb16)
     bytes_px_32 = sum(stage_bytes(16, 8, p32.iterations, batched=False))
 
+    progress("the reference's own published case")
     # ---- the reference's own published case: REFERENCE mode, rgba16f, 1280x720, init + 3 passes ----
     ref720 = None
     if args.ref720:
@@ -679,14 +692,32 @@ def main():
                     rctx.denoise(radiance=rad, denoised=den, moments_curr=mom[k % 2], moments_prev=mom[(k + 1) % 2])
                 e1.record(stream)
                 torch.cuda.synchronize()
+            # the reference's own instrument: per-pass start / end timestamps, 2 frames in
+            # flight, EMA (alpha 0.05) and a log line every 60 frames (src/main.cpp:2902-2953)
+            prof_frames = 180
+            with S.Profiler(["svgf"], device=0) as prof, torch.cuda.stream(stream):
+                for k in range(prof_frames):
+                    prof.begin_frame()
+                    prof.pass_begin(0, stream)
+                    rctx.denoise(radiance=rad, denoised=den, moments_curr=mom[k % 2], moments_prev=mom[(k + 1) % 2])
+                    prof.pass_end(0, stream)
+                    prof.end_frame()
+                torch.cuda.synchronize()
+                prof_ema_ms = float(prof.ema()[0]) / 1e6
+                prof_line = prof.log_line()
         ms = e0.elapsed_time(e1) / args.steps
         ref720 = {"what": "REFERENCE mode (svgf_denoise.slang semantics), rgba16f/rg16f, 1280x720, init + 3 a-trous "
                           "passes per frame, back-to-back frames",
+                  "profiler": {"what": f"svgf_profiler over {prof_frames} frames: hipEvents around each frame's "
+                                       "denoise, 2 frames in flight, EMA alpha 0.05, log every 60 frames (the "
+                                       "reference's GPU profiler, src/main.cpp:145-180, 2902-2953)",
+                               "ema_ms": round(prof_ema_ms, 5), "last_log_line": prof_line},
                   "ms_per_frame": round(ms, 5), "mpx_per_s": round(w * h / ms / 1e3, 1),
                   "reference_published_ms": 0.71,
                   "reference_published_source": "TODO.md:1170-1172 (debug build, GPU not named; dev GPU RTX 4090 Laptop)",
                   "speedup_vs_published": round(0.71 / ms, 2)}
 
+    progress("serving form")
     # ---- serving form: Poisson arrivals from 256 agent streams, continuous batching ----
     serving = None
     if args.serve_seconds > 0:
@@ -711,13 +742,16 @@ def main():
                                    rate_hz=args.serve_rate, duration_s=args.serve_seconds, churn_per_s=0.05, seed=11)
         sctx.close()
 
+    progress("agent frames")
     # ---- agent frames: batched SVGF at eye sizes + wire encode + readback, one tick ----
     agent = agent_report(args.agents, args.agent_ticks, args.warmup, not args.no_cpu_baseline) if args.agents > 0 else None
 
+    progress("consumer composite")
     # ---- consumer composite: the reader of the denoised output ----
     composite = composite_report(args.composite_reps, not args.no_cpu_baseline) if args.composite_reps > 0 else None
     pack = pack_report(args.composite_reps, not args.no_cpu_baseline) if args.composite_reps > 0 else None
 
+    progress("CPU baseline")
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----
     cpu = None
     if not args.no_cpu_baseline:

@@ -442,6 +442,48 @@ SVGF_API svgf_status svgf_pack_destroy(svgf_pack* pack);
 /* Enqueues the pack of n frames on hip_stream (asynchronous). Outputs may not overlap inputs. */
 SVGF_API svgf_status svgf_pack_run(svgf_pack* pack, const svgf_pack_frame* frames, uint32_t n, void* hip_stream);
 
+/* --- frame profiler ---------------------------------------------------------------------------
+ * TronGrid's per-pass GPU profiler (src/main.cpp:145-180, 1180-1207, 2902-2953) on hipEvents:
+ * every pass of a frame is bracketed by a start and an end event recorded into the frame's slot
+ * (`slots` frames in flight, MAX_FRAMES_IN_FLIGHT = 2); when a slot comes round again its
+ * events are read back (waiting for them, as the reference waits for the slot's fence) and each
+ * pass's duration updates an exponential moving average,
+ *   first readback: ema = sample;  later: ema = ema * (1 - alpha) + sample * alpha   (float)
+ * and when the frame counter at that readback is a multiple of log_interval the line
+ *   "GPU times (ms): <name>=<ema / 1e6 as %f> ..." is composed. The frame counter starts at
+ * first_frame (WARMUP_ITERATIONS = 16) and advances once per svgf_profiler_begin_frame.
+ * Per frame: svgf_profiler_begin_frame, then pass_begin / pass_end pairs around the work, then
+ * svgf_profiler_end_frame. A pass without both events in a frame samples 0.
+ * device = -1 makes a host-only profiler for replaying recorded durations: samples are given
+ * with svgf_profiler_sample instead of events. */
+typedef struct svgf_profiler svgf_profiler;
+
+typedef struct svgf_profiler_desc {
+    uint32_t passes;          /* 1..32 */
+    uint32_t slots;           /* frames in flight, 1..8; reference 2 (main.cpp:74) */
+    float ema_alpha;          /* reference GPU_PROFILER_EMA_ALPHA = 0.05 (main.cpp:170) */
+    uint32_t log_interval;    /* frames; reference GPU_PROFILER_LOG_INTERVAL_FRAMES = 60 (main.cpp:175) */
+    uint32_t first_frame;     /* initial frame counter; reference WARMUP_ITERATIONS = 16 (main.cpp:2611) */
+    int32_t device;           /* HIP device ordinal, or -1 for host samples only */
+    const char* const* names; /* `passes` names (copied, at most 31 characters each); NULL: pass0, pass1, ... */
+} svgf_profiler_desc;
+
+SVGF_API svgf_status svgf_profiler_default_desc(svgf_profiler_desc* out);
+SVGF_API svgf_status svgf_profiler_create(const svgf_profiler_desc* desc, svgf_profiler** out);
+SVGF_API svgf_status svgf_profiler_destroy(svgf_profiler* prof);
+/* Reads back the slot this frame reuses (if it holds a frame), updates the averages and composes
+ * the log line when due (*logged = 1, else 0; logged may be NULL); then advances the counter. */
+SVGF_API svgf_status svgf_profiler_begin_frame(svgf_profiler* prof, int32_t* logged);
+SVGF_API svgf_status svgf_profiler_pass_begin(svgf_profiler* prof, uint32_t pass, void* hip_stream);
+SVGF_API svgf_status svgf_profiler_pass_end(svgf_profiler* prof, uint32_t pass, void* hip_stream);
+/* Host-only profiler: the current frame's duration of `pass`, in nanoseconds. */
+SVGF_API svgf_status svgf_profiler_sample(svgf_profiler* prof, uint32_t pass, float ns);
+SVGF_API svgf_status svgf_profiler_end_frame(svgf_profiler* prof);
+/* Averages in nanoseconds (count = passes; SVGF_ERR_NOT_READY before the first readback). */
+SVGF_API svgf_status svgf_profiler_ema(svgf_profiler* prof, float* ema_ns, uint32_t capacity, uint32_t* count);
+/* The last composed log line (empty before the first), NUL-terminated, truncated to capacity. */
+SVGF_API svgf_status svgf_profiler_log_line(svgf_profiler* prof, char* buf, uint32_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,8 @@ def lib():
                                        ctypes.c_int, fp]
         up = ctypes.POINTER(ctypes.c_uint32)
         L.oracle_unpack_octahedral.argtypes = [ctypes.c_uint32, fp]
+        L.oracle_profiler_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, ctypes.c_float, ctypes.c_uint32,
+                                          ctypes.c_uint32, fp, ctypes.POINTER(ctypes.c_uint8)]
         L.oracle_pack.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, fp, up, fp, fp, fp, ctypes.c_int, fp, fp, fp]
         _lib = L
     return _lib
@@ -289,3 +291,17 @@ def pack(vp, pvp, position, normal_oct, indirect, direct_diffuse, direct_specula
     if rc:
         raise ValueError(f"oracle_pack failed: {rc}")
     return rad, g, mv
+
+
+def profiler_run(samples, slots=2, alpha=0.05, interval=60, first=16):
+    """The reference's profiler loop over per-iteration pass samples (frames, passes) in ns:
+    returns (ema (frames, passes) float32 after each iteration's readback, logged (frames,) bool)."""
+    samples = _f32(samples)
+    frames, passes = samples.shape
+    ema = np.zeros((frames, passes), np.float32)
+    logged = np.zeros(frames, np.uint8)
+    rc = lib().oracle_profiler_run(passes, slots, frames, _fp(samples), float(alpha), interval, first, _fp(ema),
+                                   logged.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    if rc:
+        raise ValueError(f"oracle_profiler_run failed: {rc}")
+    return ema, logged.astype(bool)

@@ -111,6 +111,11 @@ int oracle_pack(int w, int h, const float* vp, const float* pvp, const float* po
                 const float* indirect, const float* direct_diffuse, const float* direct_specular, int store,
                 float* radiance, float* gbuffer, float* motion);
 
+/* The reference's GPU pass profiler loop (profiler_oracle.c; main.cpp:2910-2951, 3043, 3096,
+ * 3148). samples, ema: frames*passes floats (ns); logged: frames bytes. Returns 0 on success. */
+int oracle_profiler_run(int passes, int slots, int frames, const float* samples, float alpha, uint32_t interval,
+                        uint32_t first, float* ema, uint8_t* logged);
+
 #ifdef __cplusplus
 }
 #endif

@@ -254,3 +254,12 @@ def test_pack_argument_checks_without_gpu():
         gpu = False
     if not gpu:
         assert L.svgf_pack_create(0, 16, ctypes.byref(h)) == 8
+
+
+def test_profiler_host_mode_works_without_gpu():
+    with S.Profiler(["svgf"], device=-1) as p:
+        for k in range(3):
+            p.begin_frame()
+            p.sample(0, 1000.0 * (k + 1))
+            p.end_frame()
+        assert p.ema()[0] == 1000.0  # frame 0 read back at frame 2 (two slots)

@@ -127,6 +127,12 @@ class PackFrame(ctypes.Structure):
                     (n, Plane) for n in PACK_INPUTS + PACK_OUTPUTS]
 
 
+class ProfilerDesc(ctypes.Structure):
+    _fields_ = [("passes", ctypes.c_uint32), ("slots", ctypes.c_uint32), ("ema_alpha", ctypes.c_float),
+                ("log_interval", ctypes.c_uint32), ("first_frame", ctypes.c_uint32), ("device", ctypes.c_int32),
+                ("names", ctypes.POINTER(ctypes.c_char_p))]
+
+
 class CompositeParams(ctypes.Structure):
     _fields_ = [("ambient", ctypes.c_float), ("reserved", ctypes.c_uint32 * 3)]
 
@@ -242,6 +248,16 @@ def lib():
             "svgf_pack_create": [ctypes.c_int32, ctypes.c_uint32, P(vp)],
             "svgf_pack_destroy": [vp],
             "svgf_pack_run": [vp, P(PackFrame), ctypes.c_uint32, vp],
+            "svgf_profiler_default_desc": [P(ProfilerDesc)],
+            "svgf_profiler_create": [P(ProfilerDesc), P(vp)],
+            "svgf_profiler_destroy": [vp],
+            "svgf_profiler_begin_frame": [vp, P(ctypes.c_int32)],
+            "svgf_profiler_pass_begin": [vp, ctypes.c_uint32, vp],
+            "svgf_profiler_pass_end": [vp, ctypes.c_uint32, vp],
+            "svgf_profiler_sample": [vp, ctypes.c_uint32, ctypes.c_float],
+            "svgf_profiler_end_frame": [vp],
+            "svgf_profiler_ema": [vp, P(ctypes.c_float), ctypes.c_uint32, P(ctypes.c_uint32)],
+            "svgf_profiler_log_line": [vp, ctypes.c_char_p, ctypes.c_uint32],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args
@@ -718,3 +734,72 @@ class Pack:
     def run(self, items, stream=None, frames=None):
         arr = frames if frames is not None else self.frames(items)
         check(lib().svgf_pack_run(self.handle, arr, len(arr), _stream_handle(stream)), "svgf_pack_run")
+
+
+# ---------------------------------------------------------------------------------------------
+# frame profiler (the reference's per-pass EMA + periodic log, on hipEvents)
+# ---------------------------------------------------------------------------------------------
+class Profiler:
+    """svgf_profiler_*: per-pass start / end events per frame slot, EMA of the durations, a log
+    line every `log_interval` frames. device=-1: host samples only (sample())."""
+
+    def __init__(self, names=("svgf",), slots=2, ema_alpha=0.05, log_interval=60, first_frame=16, device=0):
+        d = ProfilerDesc()
+        check(lib().svgf_profiler_default_desc(ctypes.byref(d)), "svgf_profiler_default_desc")
+        self._names = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+        d.passes, d.slots, d.ema_alpha, d.log_interval = len(names), slots, ema_alpha, log_interval
+        d.first_frame, d.device = first_frame, device
+        d.names = ctypes.cast(self._names, ctypes.POINTER(ctypes.c_char_p))
+        self.passes = len(names)
+        self.handle = ctypes.c_void_p()
+        check(lib().svgf_profiler_create(ctypes.byref(d), ctypes.byref(self.handle)), "svgf_profiler_create")
+
+    def close(self):
+        if self.handle:
+            check(lib().svgf_profiler_destroy(self.handle), "svgf_profiler_destroy")
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def begin_frame(self) -> bool:
+        logged = ctypes.c_int32()
+        check(lib().svgf_profiler_begin_frame(self.handle, ctypes.byref(logged)), "svgf_profiler_begin_frame")
+        return bool(logged.value)
+
+    def pass_begin(self, k, stream=None):
+        check(lib().svgf_profiler_pass_begin(self.handle, k, _stream_handle(stream)), "svgf_profiler_pass_begin")
+
+    def pass_end(self, k, stream=None):
+        check(lib().svgf_profiler_pass_end(self.handle, k, _stream_handle(stream)), "svgf_profiler_pass_end")
+
+    def sample(self, k, ns):
+        check(lib().svgf_profiler_sample(self.handle, k, float(ns)), "svgf_profiler_sample")
+
+    def end_frame(self):
+        check(lib().svgf_profiler_end_frame(self.handle), "svgf_profiler_end_frame")
+
+    def ema(self):
+        """Per-pass averages in ns (numpy float32), or None before the first readback."""
+        out = np.zeros(self.passes, np.float32)
+        n = ctypes.c_uint32()
+        st = lib().svgf_profiler_ema(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), self.passes,
+                                     ctypes.byref(n))
+        if st == 10:
+            return None
+        check(st, "svgf_profiler_ema")
+        return out
+
+    def log_line(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().svgf_profiler_log_line(self.handle, buf, len(buf)), "svgf_profiler_log_line")
+        return buf.value.decode()
