@@ -149,7 +149,7 @@ int main(int argc, char** argv)
             uint32_t t = 0;
             for (int st = 0; st < stages; ++st)
                 for (uint32_t f = 0; f < n; ++f) {
-                    const uint32_t nt = tiles_at(w[f], h[f], stage_log2s(canonical, st));
+                    const uint32_t nt = stage_tiles(canonical, st, w[f], h[f]);
                     if (frame_tiles(B.starts.data(), (uint32_t)st, f) != nt) fail("frame_tiles", (uint32_t)st, f, nt);
                     for (uint32_t i = 0; i < nt; ++i, ++t) {
                         const Ticket k = decode_ticket(B.hdr, B.starts.data(), t);

@@ -211,6 +211,14 @@ __device__ __forceinline__ Tile decode_tile(int w, uint32_t log2s, uint32_t t)
     return Tile{(int)log2s, (int)(res & (s - 1)), (int)(res >> log2s), (int)(tx * LT), (int)(ty * LT)};
 }
 
+// A wide tile of the variance stage: VAR_TX lattice tiles side by side (stride 1).
+__device__ __forceinline__ Tile decode_wide_tile(int w, uint32_t t)
+{
+    const uint32_t ntx = ((uint32_t)w + LT * VAR_TX - 1) / (LT * VAR_TX);
+    const uint32_t ty = t / ntx, tx = t - ty * ntx;
+    return Tile{0, 0, 0, (int)(tx * LT * VAR_TX), (int)(ty * LT)};
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Renumber so
 // that XCD x runs one contiguous range of logical tiles: neighbouring tiles, which share
 // halo rows and (for s > 1) cache lines of other residues, then meet in the same L2.
@@ -286,6 +294,8 @@ __device__ __forceinline__ void ref_atrous_tile(const FrameDesc& f, const KParam
                 s2 = fmaf(v.z, tw, s2);
                 ws += tw;
             }
+            // the row's sums are complete here (keeps later rows' weights from staying live)
+            asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(ws));
         }
         float4 o;
         if (ws > 1e-6f) o = make_float4(s0 / ws, s1 / ws, s2 / ws, cw);
@@ -464,65 +474,72 @@ __device__ __forceinline__ float edge_log2(const KParams& p, float4 Gc, float4 G
 // place with the edge-aware 3x3 variance of the temporal luminance. Older pixels keep the
 // temporal variance the temporal stage wrote. (Only the alpha bytes are written; neighbours
 // read only colour bytes, so the in-place update has no read/write overlap.)
+// T is a wide tile (VAR_TX 16x16 tiles side by side): young pixels are scattered
+// (disocclusions), so the wide tile's young pixels are compacted first and the block's threads
+// run the 3x3 window for them only.
 template <class S>
 __device__ __forceinline__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
-    // Young pixels are scattered (disocclusions), so nearly every wave holds a few of them.
-    // Compact them first: the tile's young pixels go to the leading lanes of the block, and
-    // only ceil(count / 64) waves run the 3x3 window.
-    __shared__ int sList[BLOCK];
+    __shared__ int sList[BLOCK * VAR_TX];
     __shared__ int sCount;
     const int tid = lane_tid();
+    if (tid == 0) sCount = 0;
+    lds_writes_done();
+    __syncthreads();
     {
-        const int x = T.i0 + (tid & (LT - 1)), y = T.j0 + (tid >> 4);
-        bool young = false;
-        if (x < f.w && y < f.h) {
-            const Buf nl = bind(f.nlen, f.h);
-            young = ld_u8(nl, y * nl.pitch + x) < 4u;
+        const Buf nl = bind(f.nlen, f.h);
+        const int y = T.j0 + (tid >> 4);
+        bool young[VAR_TX];
+#pragma unroll
+        for (int j = 0; j < VAR_TX; ++j) {
+            const int x = T.i0 + j * LT + (tid & (LT - 1));
+            young[j] = x < f.w && y < f.h && ld_u8(nl, y * nl.pitch + x) < 4u;
         }
-        if (tid == 0) sCount = 0;
-        lds_writes_done();
-        __syncthreads();
-        if (young) sList[atomicAdd(&sCount, 1)] = tid;
-        lds_writes_done();
-        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < VAR_TX; ++j)
+            if (young[j]) sList[atomicAdd(&sCount, 1)] = j * BLOCK + tid;
     }
-    if (tid >= sCount) return;
-    const int src_lane = sList[tid];
-    const int x = T.i0 + (src_lane & (LT - 1)), y = T.j0 + (src_lane >> 4);
+    lds_writes_done();
+    __syncthreads();
+    const int count = sCount;
     const Buf gb = bind(f.g, f.h), wa = bind(f.wa, f.h), gr = bind(f.grad, f.h);
-    float4 G = ldg4(gb, x, y);
-    const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
-    const float gx = gxy.x, gy = gxy.y;
-    float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
+    for (int i = tid; i < count; i += BLOCK) {
+        const int e = sList[i];
+        const int lane = e & (BLOCK - 1);
+        const int x = T.i0 + (e / BLOCK) * LT + (lane & (LT - 1)), y = T.j0 + (lane >> 4);
+        float4 G = ldg4(gb, x, y);
+        const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
+        const float gx = gxy.x, gy = gxy.y;
+        float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-        int qy = clampi(y + dy, 0, f.h - 1);
+        for (int dy = -1; dy <= 1; ++dy) {
+            int qy = clampi(y + dy, 0, f.h - 1);
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-            int qx = clampi(x + dx, 0, f.w - 1);
-            int t = (dy + 1) * 3 + dx + 1;
-            float4 Gq = ldg4(gb, qx, qy);
-            float w = __builtin_amdgcn_exp2f(edge_log2(p, G, Gq, gx, gy, (float)dx, (float)dy));
-            float4 Cq = WavS::ld4(wa, qx, qy);
-            tw[t] = w;
-            tl[t] = luma_fma(Cq.x, Cq.y, Cq.z);
-            sw += w;
-            s1 = fmaf(w, tl[t], s1);
+            for (int dx = -1; dx <= 1; ++dx) {
+                int qx = clampi(x + dx, 0, f.w - 1);
+                int t = (dy + 1) * 3 + dx + 1;
+                float4 Gq = ldg4(gb, qx, qy);
+                float w = __builtin_amdgcn_exp2f(edge_log2(p, G, Gq, gx, gy, (float)dx, (float)dy));
+                float4 Cq = WavS::ld4(wa, qx, qy);
+                tw[t] = w;
+                tl[t] = luma_fma(Cq.x, Cq.y, Cq.z);
+                sw += w;
+                s1 = fmaf(w, tl[t], s1);
+            }
         }
-    }
-    float var = 0.0f;
-    if (sw > 0.0f) {
-        float inv = rcp_hw(sw);
-        float mean = s1 * inv, s2 = 0.0f;
+        float var = 0.0f;
+        if (sw > 0.0f) {
+            float inv = rcp_hw(sw);
+            float mean = s1 * inv, s2 = 0.0f;
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            float d = tl[t] - mean;
-            s2 = fmaf(tw[t] * d, d, s2);
+            for (int t = 0; t < 9; ++t) {
+                float d = tl[t] - mean;
+                s2 = fmaf(tw[t] * d, d, s2);
+            }
+            var = s2 * inv;
         }
-        var = s2 * inv;
+        WavS::sta(wa, x, y, sqrt_hw(var));
     }
-    WavS::sta(wa, x, y, sqrt_hw(var));
 }
 
 // A-trous iteration k (stride s = 2^k) on one lattice tile: luminance, depth and normal
@@ -593,6 +610,10 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
                 sw += w;
                 sv = fmaf(w * w, sV[q], sv);
             }
+            // The row's sums are complete here. Without this the optimiser sinks the weighted
+            // sums below later rows' exponents and keeps those taps' weights and colours live:
+            // 77 VGPRs instead of 58, 6 waves per SIMD instead of 8, and a quarter slower.
+            asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(sw), "+v"(sv));
         }
         float4 o;
         if (sw > 1e-6f) {
@@ -638,11 +659,19 @@ __device__ __forceinline__ FrameDesc load_frame(const FrameDesc* frames, uint32_
     return f;
 }
 
+// The tile of stage STAGE that logical workgroup t of a frame runs.
+template <int STAGE>
+__device__ __forceinline__ Tile stage_tile(int w, uint32_t log2s, uint32_t t)
+{
+    if constexpr (STAGE == ST_CAN_VARIANCE) return decode_wide_tile(w, t);
+    else return decode_tile(w, log2s, t);
+}
+
 // arg: REFERENCE a-trous pass (1..N) or CANONICAL a-trous iteration (0..N-1); log2s: stride.
 template <class S, int STAGE, bool COMP>
 __global__ __launch_bounds__(BLOCK) void k_frame(FrameDesc f, KParams p, int arg, uint32_t log2s)
 {
-    Tile T = decode_tile(f.w, log2s, xcd_logical(blockIdx.x, gridDim.x));
+    Tile T = stage_tile<STAGE>(f.w, log2s, xcd_logical(blockIdx.x, gridDim.x));
     run_stage<S, STAGE, COMP>(f, p, T, arg);
 }
 
@@ -657,7 +686,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchTable bt, KParams p, int a
     if (t >= starts[nf]) return;
     const uint32_t fi = frame_of(starts, nf, t);
     const FrameDesc f = load_frame(bt.frames, fi);
-    Tile T = decode_tile(f.w, log2s, t - starts[fi]);
+    Tile T = stage_tile<STAGE>(f.w, log2s, t - starts[fi]);
     run_stage<S, STAGE, COMP>(f, p, T, arg);
 }
 
@@ -669,7 +698,7 @@ __device__ __forceinline__ void persist_tile(const FrameDesc* frames, uint32_t f
                                           uint32_t tile, int arg)
 {
     const FrameDesc f = load_frame(frames, fi);
-    run_stage<S, STAGE, COMP>(f, p, decode_tile(f.w, log2s, tile), arg);
+    run_stage<S, STAGE, COMP>(f, p, stage_tile<STAGE>(f.w, log2s, tile), arg);
 }
 
 // Persistent form: every stage of a batch in one launch. Each workgroup takes chunks of
@@ -799,7 +828,8 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
 template <class S, int STAGE, bool COMP>
 void launch_frame(const FrameDesc& f, const KParams& p, int arg, uint32_t log2s, hipStream_t stream)
 {
-    uint32_t n = tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s);
+    uint32_t n = STAGE == ST_CAN_VARIANCE ? wide_tiles((uint32_t)f.w, (uint32_t)f.h)
+                                          : tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s);
     hipLaunchKernelGGL((k_frame<S, STAGE, COMP>), dim3(n), dim3(BLOCK), 0, stream, f, p, arg, log2s);
 }
 

@@ -40,6 +40,23 @@ SVGF_HD inline uint32_t stage_log2s(int canonical, int stage)
     int first_atrous = canonical ? 2 : 1;
     return stage < first_atrous ? 0u : (uint32_t)(stage - first_atrous);
 }
+// The CANONICAL variance stage (stage 1) reads one byte per pixel except where the history is
+// young, so it runs on wider tiles: VAR_TX lattice tiles side by side (64x16 pixels) per
+// workgroup, which spreads a workgroup's fixed cost (dispatch, two barriers) over 4x the pixels.
+constexpr int VAR_TX = 4;
+SVGF_HD inline bool is_variance_stage(int canonical, int stage) { return canonical && stage == 1; }
+
+SVGF_HD inline uint32_t wide_tiles(uint32_t w, uint32_t h)
+{
+    return ((w + LT * VAR_TX - 1) / (LT * VAR_TX)) * ((h + LT - 1) / LT);
+}
+
+// Tiles (workgroups) of stage `stage` for a w x h frame.
+SVGF_HD inline uint32_t stage_tiles(int canonical, int stage, uint32_t w, uint32_t h)
+{
+    return is_variance_stage(canonical, stage) ? wide_tiles(w, h) : tiles_at(w, h, stage_log2s(canonical, stage));
+}
+
 constexpr int MAX_STAGES = 12;
 constexpr uint32_t MAX_BATCH_FRAMES = 1024;
 constexpr uint32_t STARTS_ROW = MAX_BATCH_FRAMES + 1;
@@ -105,7 +122,7 @@ inline uint64_t build_schedule(uint32_t n, SizeOf size, int canonical, int stage
             uint32_t w = 0, h = 0;
             size(i, w, h);
             row[i] = tile;
-            tile += tiles_at(w, h, stage_log2s(canonical, st));
+            tile += stage_tiles(canonical, st, w, h);
         }
         row[n] = tile;
         hdr[HDR_BASE + st] = (uint32_t)ticket;
