@@ -474,14 +474,20 @@ __device__ __forceinline__ float edge_log2(const KParams& p, float4 Gc, float4 G
 // place with the edge-aware 3x3 variance of the temporal luminance. Older pixels keep the
 // temporal variance the temporal stage wrote. (Only the alpha bytes are written; neighbours
 // read only colour bytes, so the in-place update has no read/write overlap.)
-// T is a wide tile (VAR_TX 16x16 tiles side by side): young pixels are scattered
-// (disocclusions), so the wide tile's young pixels are compacted first and the block's threads
-// run the 3x3 window for them only.
+// T is a wide tile (VAR_TX 16x16 tiles side by side). Young pixels are scattered
+// (disocclusions): the tile's young pixels are listed first, and when there are any, the
+// tile's G-buffer and temporal luminance plus a 1-pixel ring are staged in LDS with coalesced
+// loads, so each young pixel's 3x3 window costs LDS reads instead of 18 scattered loads.
+constexpr int VW = LT * VAR_TX + 2;  // 66 staged columns
+constexpr int VH = LT + 2;           // 18 staged rows
+
 template <class S>
 __device__ __forceinline__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
     __shared__ int sList[BLOCK * VAR_TX];
     __shared__ int sCount;
+    __shared__ float4 sG[VW * VH];  // G-buffer of the tile + ring
+    __shared__ float sLu[VW * VH];  // temporal luminance of the tile + ring
     const int tid = lane_tid();
     if (tid == 0) sCount = 0;
     lds_writes_done();
@@ -502,27 +508,38 @@ __device__ __forceinline__ void can_variance_tile(const FrameDesc& f, const KPar
     lds_writes_done();
     __syncthreads();
     const int count = sCount;
+    if (count == 0) return;  // uniform: read after the barrier
     const Buf gb = bind(f.g, f.h), wa = bind(f.wa, f.h), gr = bind(f.grad, f.h);
+    // entry (li, lj) holds pixel (clamp(i0 - 1 + li), clamp(j0 - 1 + lj)); for an in-frame
+    // pixel x, entry x + dx - (i0 - 1) then holds clamp(x + dx), the window's clamped tap
+    for (int e = tid; e < VW * VH; e += BLOCK) {
+        const int li = e % VW, lj = e / VW;
+        const int qx = clampi(T.i0 - 1 + li, 0, f.w - 1), qy = clampi(T.j0 - 1 + lj, 0, f.h - 1);
+        sG[e] = ldg4(gb, qx, qy);
+        const float4 c = WavS::ld4(wa, qx, qy);
+        sLu[e] = luma_fma(c.x, c.y, c.z);
+    }
+    lds_writes_done();
+    __syncthreads();
     for (int i = tid; i < count; i += BLOCK) {
         const int e = sList[i];
         const int lane = e & (BLOCK - 1);
-        const int x = T.i0 + (e / BLOCK) * LT + (lane & (LT - 1)), y = T.j0 + (lane >> 4);
-        float4 G = ldg4(gb, x, y);
+        const int bx = (e / BLOCK) * LT + (lane & (LT - 1)), by = lane >> 4;  // in the wide tile
+        const int x = T.i0 + bx, y = T.j0 + by;
+        const int c0 = (by + 1) * VW + bx + 1;
+        const float4 G = sG[c0];
         const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
         const float gx = gxy.x, gy = gxy.y;
         float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
 #pragma unroll
         for (int dy = -1; dy <= 1; ++dy) {
-            int qy = clampi(y + dy, 0, f.h - 1);
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx) {
-                int qx = clampi(x + dx, 0, f.w - 1);
-                int t = (dy + 1) * 3 + dx + 1;
-                float4 Gq = ldg4(gb, qx, qy);
-                float w = __builtin_amdgcn_exp2f(edge_log2(p, G, Gq, gx, gy, (float)dx, (float)dy));
-                float4 Cq = WavS::ld4(wa, qx, qy);
+                const int t = (dy + 1) * 3 + dx + 1;
+                const int q = c0 + dy * VW + dx;
+                float w = __builtin_amdgcn_exp2f(edge_log2(p, G, sG[q], gx, gy, (float)dx, (float)dy));
                 tw[t] = w;
-                tl[t] = luma_fma(Cq.x, Cq.y, Cq.z);
+                tl[t] = sLu[q];
                 sw += w;
                 s1 = fmaf(w, tl[t], s1);
             }
