@@ -213,7 +213,8 @@ SVGF_API svgf_status svgf_denoise_batch(svgf_context* ctx, const svgf_request* r
  * Needs SVGF_PROFILE_STAGES in svgf_desc.batch_flags (hipEvents around each stage launch, the
  * counterpart of the reference's timestamp pair around the SVGF chain, main.cpp:907, 947).
  * Waits for that call to finish. Stage order: REFERENCE init, a-trous 0..N-1; CANONICAL
- * temporal, variance, a-trous 0..N-1. *count receives the number of stages. */
+ * temporal, a-trous 0..N-1 (a-trous 0 also computes the spatial variance of young pixels).
+ * *count receives the number of stages. */
 SVGF_API svgf_status svgf_stage_times(svgf_context* ctx, float* ms, uint32_t capacity, uint32_t* count);
 
 /* --- serving form: continuous batching over resident streams -----------------------------

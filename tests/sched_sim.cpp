@@ -130,7 +130,7 @@ int main(int argc, char** argv)
         const uint32_t n = b == 0 ? 1u : 1u + rng() % (b % 5 == 0 ? 200u : 24u);
         const int canonical = (int)(rng() & 1);
         const int iterations = 1 + (int)(rng() % 5);
-        const int stages = iterations + (canonical ? 2 : 1);
+        const int stages = iterations + 1;
         std::vector<uint32_t> w(n), h(n);
         for (uint32_t i = 0; i < n; ++i) {
             // mostly small frames, so that the simulation stays fast; 1280x720 now and then

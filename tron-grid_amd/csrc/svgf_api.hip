@@ -214,7 +214,7 @@ void destroy_context(svgf_context* c)
     delete c;
 }
 
-int stage_count(const svgf_context* c) { return c->kp.iterations + (c->mode == SVGF_MODE_REFERENCE ? 1 : 2); }
+int stage_count(const svgf_context* c) { return c->kp.iterations + 1; }
 
 // One persistent launch per batch (stage profiling needs one launch per stage).
 bool persistent(const svgf_context* c)

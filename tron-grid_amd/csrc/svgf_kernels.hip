@@ -211,14 +211,6 @@ __device__ __forceinline__ Tile decode_tile(int w, uint32_t log2s, uint32_t t)
     return Tile{(int)log2s, (int)(res & (s - 1)), (int)(res >> log2s), (int)(tx * LT), (int)(ty * LT)};
 }
 
-// A wide tile of the variance stage: VAR_TX lattice tiles side by side (stride 1).
-__device__ __forceinline__ Tile decode_wide_tile(int w, uint32_t t)
-{
-    const uint32_t ntx = ((uint32_t)w + LT * VAR_TX - 1) / (LT * VAR_TX);
-    const uint32_t ty = t / ntx, tx = t - ty * ntx;
-    return Tile{0, 0, 0, (int)(tx * LT * VAR_TX), (int)(ty * LT)};
-}
-
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Renumber so
 // that XCD x runs one contiguous range of logical tiles: neighbouring tiles, which share
 // halo rows and (for s > 1) cache lines of other residues, then meet in the same L2.
@@ -470,121 +462,115 @@ __device__ __forceinline__ float edge_log2(const KParams& p, float4 Gc, float4 G
     return a;
 }
 
-// Spatial variance where the history is young (n < 4): patches the variance channel of W0 in
-// place with the edge-aware 3x3 variance of the temporal luminance. Older pixels keep the
-// temporal variance the temporal stage wrote. (Only the alpha bytes are written; neighbours
-// read only colour bytes, so the in-place update has no read/write overlap.)
-// T is a wide tile (VAR_TX 16x16 tiles side by side). Young pixels are scattered
-// (disocclusions): the tile's young pixels are listed first, and when there are any, the
-// tile's G-buffer and temporal luminance plus a 1-pixel ring are staged in LDS with coalesced
-// loads, so each young pixel's 3x3 window costs LDS reads instead of 18 scattered loads.
-constexpr int VW = LT * VAR_TX + 2;  // 66 staged columns
-constexpr int VH = LT + 2;           // 18 staged rows
-
-template <class S>
-__device__ __forceinline__ void can_variance_tile(const FrameDesc& f, const KParams& p, const Tile& T)
+// Spatial variance of a young pixel (history length n < 4): the edge-aware variance of the
+// temporal luminance over its 3x3 window (clamped at the frame border), weighted by the depth /
+// normal edge-stops of the centre. tap(dx, dy) gives the LDS entry holding the window pixel.
+template <class Tap>
+__device__ __forceinline__ float young_variance(const KParams& p, const float4* sA, const float4* sB, float4 G,
+                                                float gx, float gy, Tap tap)
 {
-    __shared__ int sList[BLOCK * VAR_TX];
-    __shared__ int sCount;
-    __shared__ float4 sG[VW * VH];  // G-buffer of the tile + ring
-    __shared__ float sLu[VW * VH];  // temporal luminance of the tile + ring
-    const int tid = lane_tid();
-    if (tid == 0) sCount = 0;
-    lds_writes_done();
-    __syncthreads();
-    {
-        const Buf nl = bind(f.nlen, f.h);
-        const int y = T.j0 + (tid >> 4);
-        bool young[VAR_TX];
+    float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
 #pragma unroll
-        for (int j = 0; j < VAR_TX; ++j) {
-            const int x = T.i0 + j * LT + (tid & (LT - 1));
-            young[j] = x < f.w && y < f.h && ld_u8(nl, y * nl.pitch + x) < 4u;
+    for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int t = (dy + 1) * 3 + dx + 1;
+            const int q = tap(dx, dy);
+            float w = __builtin_amdgcn_exp2f(edge_log2(p, G, sB[q], gx, gy, (float)dx, (float)dy));
+            tw[t] = w;
+            tl[t] = sA[q].w;
+            sw += w;
+            s1 = fmaf(w, tl[t], s1);
         }
-#pragma unroll
-        for (int j = 0; j < VAR_TX; ++j)
-            if (young[j]) sList[atomicAdd(&sCount, 1)] = j * BLOCK + tid;
     }
-    lds_writes_done();
-    __syncthreads();
-    const int count = sCount;
-    if (count == 0) return;  // uniform: read after the barrier
-    const Buf gb = bind(f.g, f.h), wa = bind(f.wa, f.h), gr = bind(f.grad, f.h);
-    // entry (li, lj) holds pixel (clamp(i0 - 1 + li), clamp(j0 - 1 + lj)); for an in-frame
-    // pixel x, entry x + dx - (i0 - 1) then holds clamp(x + dx), the window's clamped tap
-    for (int e = tid; e < VW * VH; e += BLOCK) {
-        const int li = e % VW, lj = e / VW;
-        const int qx = clampi(T.i0 - 1 + li, 0, f.w - 1), qy = clampi(T.j0 - 1 + lj, 0, f.h - 1);
-        sG[e] = ldg4(gb, qx, qy);
-        const float4 c = WavS::ld4(wa, qx, qy);
-        sLu[e] = luma_fma(c.x, c.y, c.z);
-    }
-    lds_writes_done();
-    __syncthreads();
-    for (int i = tid; i < count; i += BLOCK) {
-        const int e = sList[i];
-        const int lane = e & (BLOCK - 1);
-        const int bx = (e / BLOCK) * LT + (lane & (LT - 1)), by = lane >> 4;  // in the wide tile
-        const int x = T.i0 + bx, y = T.j0 + by;
-        const int c0 = (by + 1) * VW + bx + 1;
-        const float4 G = sG[c0];
-        const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
-        const float gx = gxy.x, gy = gxy.y;
-        float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
+    float var = 0.0f;
+    if (sw > 0.0f) {
+        float inv = rcp_hw(sw);
+        float mean = s1 * inv, s2 = 0.0f;
 #pragma unroll
-        for (int dy = -1; dy <= 1; ++dy) {
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int t = (dy + 1) * 3 + dx + 1;
-                const int q = c0 + dy * VW + dx;
-                float w = __builtin_amdgcn_exp2f(edge_log2(p, G, sG[q], gx, gy, (float)dx, (float)dy));
-                tw[t] = w;
-                tl[t] = sLu[q];
-                sw += w;
-                s1 = fmaf(w, tl[t], s1);
-            }
+        for (int t = 0; t < 9; ++t) {
+            float d = tl[t] - mean;
+            s2 = fmaf(tw[t] * d, d, s2);
         }
-        float var = 0.0f;
-        if (sw > 0.0f) {
-            float inv = rcp_hw(sw);
-            float mean = s1 * inv, s2 = 0.0f;
-#pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                float d = tl[t] - mean;
-                s2 = fmaf(tw[t] * d, d, s2);
-            }
-            var = s2 * inv;
-        }
-        WavS::sta(wa, x, y, sqrt_hw(var));
+        var = s2 * inv;
     }
+    return var;
 }
 
 // A-trous iteration k (stride s = 2^k) on one lattice tile: luminance, depth and normal
 // edge-stops, variance filtered by squared weights. Iteration 0 also writes the colour
 // history, the last one the output.
-template <class S>
+// FIRST (iteration 0, stride 1) first replaces the temporal variance of young pixels (n < 4)
+// by their spatial variance. The taps and the variance prefilter read the variance of every
+// entry of the 20x20 tap region, and a young entry's 3x3 window reaches one pixel further, so
+// this iteration stages a 22x22 region; the young entries of the tap region are listed and
+// the block's threads compute their windows from LDS before the filter runs.
+template <class S, bool FIRST>
 __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
 {
-    __shared__ float4 sA[LREG * LREG];  // (r, g, b, luminance)
-    __shared__ float4 sB[LREG * LREG];  // (normal, depth)
-    __shared__ float sV[LREG * LREG];   // variance
+    constexpr int H = FIRST ? LHALO + 1 : LHALO;  // staged halo
+    constexpr int R = LT + 2 * H;                 // staged region width
+    __shared__ float4 sA[R * R];  // (r, g, b, luminance)
+    __shared__ float4 sB[R * R];  // (normal, depth)
+    __shared__ float sV[R * R];   // variance
+    __shared__ int sList[FIRST ? LREG * LREG : 1];
+    __shared__ int sCount;
     const int s = 1 << k;
     const Buf src = bind((k & 1) ? f.wb : f.wa, f.h), gb = bind(f.g, f.h);
     const int max_x = f.w - 1, max_y = f.h - 1;
     const int tid = lane_tid();
-    for (int e = tid; e < LREG * LREG; e += BLOCK) {
-        int li = e % LREG, lj = e / LREG;
-        int qx = clampi(T.px(T.i0 - LHALO + li), 0, max_x), qy = clampi(T.py(T.j0 - LHALO + lj), 0, max_y);
+    // FIRST: history lengths of the tap region's entries tid and tid + 256, loaded together with
+    // the staged region (no extra round trip)
+    constexpr int NQ = FIRST ? (LREG * LREG + BLOCK - 1) / BLOCK : 0;
+    uint32_t nq[NQ > 0 ? NQ : 1];
+    if constexpr (FIRST) {
+        if (tid == 0) sCount = 0;
+        const Buf nl = bind(f.nlen, f.h);
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            const int e = tid + j * BLOCK;
+            const int qx = clampi(T.i0 - LHALO + e % LREG, 0, max_x), qy = clampi(T.j0 - LHALO + e / LREG, 0, max_y);
+            nq[j] = e < LREG * LREG ? ld_u8(nl, qy * nl.pitch + qx) : 4u;
+        }
+    }
+    for (int e = tid; e < R * R; e += BLOCK) {
+        int li = e % R, lj = e / R;
+        int qx = clampi(T.px(T.i0 - H + li), 0, max_x), qy = clampi(T.py(T.j0 - H + lj), 0, max_y);
         float4 v = WavS::ld4(src, qx, qy);
         sA[e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
         sB[e] = ldg4(gb, qx, qy);
         sV[e] = v.w * v.w;
     }
+    if constexpr (FIRST) {
+        lds_writes_done();
+        __syncthreads();  // the staged region and sCount = 0
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+            if (nq[j] < 4u) sList[atomicAdd(&sCount, 1)] = tid + j * BLOCK;
+        lds_writes_done();
+        __syncthreads();
+        const int count = sCount;
+        const Buf gr = bind(f.grad, f.h);
+        for (int i = tid; i < count; i += BLOCK) {
+            const int e = sList[i];
+            const int qx = clampi(T.i0 - LHALO + e % LREG, 0, max_x), qy = clampi(T.j0 - LHALO + e / LREG, 0, max_y);
+            // staged entry (li, lj) holds pixel (clamp(i0 - 3 + li), clamp(j0 - 3 + lj)), so
+            // pixel (clamp(qx + dx), clamp(qy + dy)) sits at its offset from (i0 - 3, j0 - 3)
+            const int ec = (qy - (T.j0 - H)) * R + (qx - (T.i0 - H));
+            const float2 gxy = ld_f2(gr, qy * gr.pitch + qx * 8);
+            const float var = young_variance(p, sA, sB, sB[ec], gxy.x, gxy.y, [&](int dx, int dy) {
+                return (clampi(qy + dy, 0, max_y) - (T.j0 - H)) * R + (clampi(qx + dx, 0, max_x) - (T.i0 - H));
+            });
+            const float sd = sqrt_hw(var);  // the value W0 would hold
+            sV[(e / LREG + 1) * R + e % LREG + 1] = sd * sd;
+        }
+        lds_writes_done();
+    }
     __syncthreads();
     const int ti = tid & (LT - 1), tj = tid >> 4;
     const int x = T.px(T.i0 + ti), y = T.py(T.j0 + tj);
     if (x <= max_x && y <= max_y) {
-        const int c0 = (tj + LHALO) * LREG + ti + LHALO;
+        const int c0 = (tj + H) * R + ti + H;
         const float4 C = sA[c0], Gc = sB[c0];
         // 3x3 binomial of the variance over the lattice neighbours (offsets +-s, in LDS) and the
         // per-pixel depth gradient the temporal stage stored
@@ -593,7 +579,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx)
-                gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c0 + dy * LREG + dx], gv);
+                gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c0 + dy * R + dx], gv);
         const Buf gr = bind(f.grad, f.h);
         const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
         const float gx = gxy.x, gy = gxy.y;
@@ -613,7 +599,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
 #pragma unroll
             for (int dx = -2; dx <= 2; ++dx) {
                 const int o = (dy + 2) * 5 + dx + 2;  // 0..24, centre 12
-                const int q = c0 + dy * LREG + dx;
+                const int q = c0 + dy * R + dx;
                 const float4 A = sA[q], B = sB[q];
                 float e = log2k(dx + 2, dy + 2);
                 if (o != 12) e = fmaf(fabsf(B.w - Gc.w), nrd[o > 12 ? o - 13 : 11 - o], e);
@@ -653,7 +639,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
 // ========================================================================================
 // Kernels
 // ========================================================================================
-enum StageId : int { ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_VARIANCE = 3, ST_CAN_ATROUS = 4 };
+enum StageId : int { ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_ATROUS0 = 3, ST_CAN_ATROUS = 4 };
 
 template <class S, int STAGE, bool COMP>
 __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, const Tile& T, int arg)
@@ -661,8 +647,8 @@ __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, 
     if constexpr (STAGE == ST_REF_INIT) ref_init_tile<S>(f, p, T);
     else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_tile<S>(f, p, T, arg);
     else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, COMP>(f, p, T);
-    else if constexpr (STAGE == ST_CAN_VARIANCE) can_variance_tile<S>(f, p, T);
-    else can_atrous_tile<S>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, true>(f, p, T, 0);
+    else can_atrous_tile<S, false>(f, p, T, arg);
 }
 
 // Frame descriptor of a batch entry into scalar registers (the index is uniform across the block).
@@ -680,8 +666,7 @@ __device__ __forceinline__ FrameDesc load_frame(const FrameDesc* frames, uint32_
 template <int STAGE>
 __device__ __forceinline__ Tile stage_tile(int w, uint32_t log2s, uint32_t t)
 {
-    if constexpr (STAGE == ST_CAN_VARIANCE) return decode_wide_tile(w, t);
-    else return decode_tile(w, log2s, t);
+    return decode_tile(w, log2s, t);
 }
 
 // arg: REFERENCE a-trous pass (1..N) or CANONICAL a-trous iteration (0..N-1); log2s: stride.
@@ -810,8 +795,8 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
         if (run) {
             if constexpr (CANON) {
                 if (k.stage == 0) persist_tile<S, ST_CAN_TEMPORAL, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
-                else if (k.stage == 1) persist_tile<S, ST_CAN_VARIANCE, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
-                else persist_tile<S, ST_CAN_ATROUS, COMP>(bt.frames, k.frame, p, k.stage - 2, k.tile, (int)k.stage - 2);
+                else if (k.stage == 1) persist_tile<S, ST_CAN_ATROUS0, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
+                else persist_tile<S, ST_CAN_ATROUS, COMP>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
             } else {
                 if (k.stage == 0) persist_tile<S, ST_REF_INIT, false>(bt.frames, k.frame, p, 0, k.tile, 0);
                 else persist_tile<S, ST_REF_ATROUS, false>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage);
@@ -845,8 +830,7 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
 template <class S, int STAGE, bool COMP>
 void launch_frame(const FrameDesc& f, const KParams& p, int arg, uint32_t log2s, hipStream_t stream)
 {
-    uint32_t n = STAGE == ST_CAN_VARIANCE ? wide_tiles((uint32_t)f.w, (uint32_t)f.h)
-                                          : tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s);
+    uint32_t n = tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s);
     hipLaunchKernelGGL((k_frame<S, STAGE, COMP>), dim3(n), dim3(BLOCK), 0, stream, f, p, arg, log2s);
 }
 
@@ -880,11 +864,11 @@ void canonical_frame(const FrameDesc& f, const KParams& p, hipStream_t s, hipEve
     mark(ev, 0, s);
     launch_frame<S, ST_CAN_TEMPORAL, COMP>(f, p, 0, 0, s);
     mark(ev, 1, s);
-    launch_frame<S, ST_CAN_VARIANCE, COMP>(f, p, 0, 0, s);
+    launch_frame<S, ST_CAN_ATROUS0, COMP>(f, p, 0, 0, s);
     mark(ev, 2, s);
-    for (int k = 0; k < p.iterations; ++k) {
+    for (int k = 1; k < p.iterations; ++k) {
         launch_frame<S, ST_CAN_ATROUS, COMP>(f, p, k, (uint32_t)k, s);
-        mark(ev, k + 3, s);
+        mark(ev, k + 2, s);
     }
 }
 
@@ -906,11 +890,11 @@ void canonical_batch(const BatchTable& bt, const KParams& p, const uint32_t* g, 
     mark(ev, 0, s);
     launch_batch<S, ST_CAN_TEMPORAL, COMP>(bt, p, 0, 0, 0, g, s);
     mark(ev, 1, s);
-    launch_batch<S, ST_CAN_VARIANCE, COMP>(bt, p, 0, 0, 1, g, s);
+    launch_batch<S, ST_CAN_ATROUS0, COMP>(bt, p, 0, 0, 1, g, s);
     mark(ev, 2, s);
-    for (int k = 0; k < p.iterations; ++k) {
-        launch_batch<S, ST_CAN_ATROUS, COMP>(bt, p, k, (uint32_t)k, k + 2, g, s);
-        mark(ev, k + 3, s);
+    for (int k = 1; k < p.iterations; ++k) {
+        launch_batch<S, ST_CAN_ATROUS, COMP>(bt, p, k, (uint32_t)k, k + 1, g, s);
+        mark(ev, k + 2, s);
     }
 }
 

@@ -33,28 +33,14 @@ SVGF_HD inline uint32_t tiles_at(uint32_t w, uint32_t h, uint32_t log2s)
     return (s * s) * ((lx + LT - 1) / LT) * ((ly + LT - 1) / LT);
 }
 
-// Stage list of a mode: REFERENCE = init + N a-trous; CANONICAL = temporal, variance,
-// N a-trous. log2 stride of stage i:
-SVGF_HD inline uint32_t stage_log2s(int canonical, int stage)
-{
-    int first_atrous = canonical ? 2 : 1;
-    return stage < first_atrous ? 0u : (uint32_t)(stage - first_atrous);
-}
-// The CANONICAL variance stage (stage 1) reads one byte per pixel except where the history is
-// young, so it runs on wider tiles: VAR_TX lattice tiles side by side (64x16 pixels) per
-// workgroup, which spreads a workgroup's fixed cost (dispatch, two barriers) over 4x the pixels.
-constexpr int VAR_TX = 4;
-SVGF_HD inline bool is_variance_stage(int canonical, int stage) { return canonical && stage == 1; }
-
-SVGF_HD inline uint32_t wide_tiles(uint32_t w, uint32_t h)
-{
-    return ((w + LT * VAR_TX - 1) / (LT * VAR_TX)) * ((h + LT - 1) / LT);
-}
-
+// Stage list of a mode: REFERENCE = init + N a-trous; CANONICAL = temporal + N a-trous (the
+// spatial variance of young pixels runs inside the first a-trous iteration). log2 stride of
+// stage i:
+SVGF_HD inline uint32_t stage_log2s(int /*canonical*/, int stage) { return stage < 1 ? 0u : (uint32_t)(stage - 1); }
 // Tiles (workgroups) of stage `stage` for a w x h frame.
 SVGF_HD inline uint32_t stage_tiles(int canonical, int stage, uint32_t w, uint32_t h)
 {
-    return is_variance_stage(canonical, stage) ? wide_tiles(w, h) : tiles_at(w, h, stage_log2s(canonical, stage));
+    return tiles_at(w, h, stage_log2s(canonical, stage));
 }
 
 constexpr int MAX_STAGES = 12;
