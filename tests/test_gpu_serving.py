@@ -48,7 +48,9 @@ def test_server_results_equal_alone(S, storage):
     outs = {}
     with S.Context(S.MODE_CANONICAL, st, max_width=maxw, max_height=maxh, max_streams=8, max_batch_frames=4,
                    max_batch_pixels=npx, batch_flags=flags) as ctx:
-        with S.Server(ctx, lanes=2, max_batch_frames=4, max_batch_pixels=npx) as srv:
+        # a partial batch waits up to 0.5 ms for more requests, so each burst of submissions is
+        # packed together however fast the GPU finishes the previous batch
+        with S.Server(ctx, lanes=2, max_batch_frames=4, max_batch_pixels=npx, max_wait_us=500) as srv:
             sids = [srv.stream_open(w, h) for w, h in SHAPES]
             order = [(i, f) for f in range(frames) for i in range(len(SHAPES))]
             rng = random.Random(5)

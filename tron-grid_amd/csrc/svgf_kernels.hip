@@ -211,6 +211,19 @@ __device__ __forceinline__ Tile decode_tile(int w, uint32_t log2s, uint32_t t)
     return Tile{(int)log2s, (int)(res & (s - 1)), (int)(res >> log2s), (int)(tx * LT), (int)(ty * LT)};
 }
 
+// Tile t of a residue-pair stage (stride s >= 4): residues (rx, ry) and (rx + 1, ry), rx even,
+// ordered as decode_tile orders single residues.
+__device__ __forceinline__ Tile decode_pair_tile(int w, uint32_t log2s, uint32_t t)
+{
+    const uint32_t s = 1u << log2s;
+    const uint32_t lx = ((uint32_t)w + s - 1) >> log2s;
+    const uint32_t ntx = (lx + LT - 1) / LT;
+    const uint32_t res = t & (s * s / 2 - 1);
+    const uint32_t tile = t >> (2 * log2s - 1);
+    const uint32_t ty = tile / ntx, tx = tile - ty * ntx;
+    return Tile{(int)log2s, (int)(2 * (res & (s / 2 - 1))), (int)(res >> (log2s - 1)), (int)(tx * LT), (int)(ty * LT)};
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Renumber so
 // that XCD x runs one contiguous range of logical tiles: neighbouring tiles, which share
 // halo rows and (for s > 1) cache lines of other residues, then meet in the same L2.
@@ -505,14 +518,18 @@ __device__ __forceinline__ float young_variance(const KParams& p, const float4* 
 // entry of the 20x20 tap region, and a young entry's 3x3 window reaches one pixel further, so
 // this iteration stages a 22x22 region; the young entries of the tap region are listed and
 // the block's threads compute their windows from LDS before the filter runs.
-template <class S, bool FIRST>
+// NR = 2: the tile holds residues (rx, ry) and (rx + 1, ry) (strides >= 8): the region of each
+// is staged (the pair's two pixels of an entry are loaded by neighbouring lanes, one cache line)
+// and every thread filters its lattice point in both.
+template <class S, bool FIRST, int NR>
 __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
 {
+    static_assert(!FIRST || NR == 1, "the first iteration runs at stride 1");
     constexpr int H = FIRST ? LHALO + 1 : LHALO;  // staged halo
     constexpr int R = LT + 2 * H;                 // staged region width
-    __shared__ float4 sA[R * R];  // (r, g, b, luminance)
-    __shared__ float4 sB[R * R];  // (normal, depth)
-    __shared__ float sV[R * R];   // variance
+    __shared__ float4 sA[NR * R * R];  // (r, g, b, luminance), residue-major
+    __shared__ float4 sB[NR * R * R];  // (normal, depth)
+    __shared__ float sV[NR * R * R];   // variance
     __shared__ int sList[FIRST ? LREG * LREG : 1];
     __shared__ int sCount;
     const int s = 1 << k;
@@ -533,13 +550,14 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             nq[j] = e < LREG * LREG ? ld_u8(nl, qy * nl.pitch + qx) : 4u;
         }
     }
-    for (int e = tid; e < R * R; e += BLOCK) {
+    for (int e2 = tid; e2 < NR * R * R; e2 += BLOCK) {
+        const int r = e2 % NR, e = e2 / NR;  // neighbouring lanes: the two residues of one entry
         int li = e % R, lj = e / R;
-        int qx = clampi(T.px(T.i0 - H + li), 0, max_x), qy = clampi(T.py(T.j0 - H + lj), 0, max_y);
+        int qx = clampi(T.px(T.i0 - H + li) + r, 0, max_x), qy = clampi(T.py(T.j0 - H + lj), 0, max_y);
         float4 v = WavS::ld4(src, qx, qy);
-        sA[e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
-        sB[e] = ldg4(gb, qx, qy);
-        sV[e] = v.w * v.w;
+        sA[r * R * R + e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
+        sB[r * R * R + e] = ldg4(gb, qx, qy);
+        sV[r * R * R + e] = v.w * v.w;
     }
     if constexpr (FIRST) {
         lds_writes_done();
@@ -568,70 +586,73 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     }
     __syncthreads();
     const int ti = tid & (LT - 1), tj = tid >> 4;
-    const int x = T.px(T.i0 + ti), y = T.py(T.j0 + tj);
-    if (x <= max_x && y <= max_y) {
-        const int c0 = (tj + H) * R + ti + H;
-        const float4 C = sA[c0], Gc = sB[c0];
-        // 3x3 binomial of the variance over the lattice neighbours (offsets +-s, in LDS) and the
-        // per-pixel depth gradient the temporal stage stored
-        float gv = 0.0f;
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) {
+        const int x = T.px(T.i0 + ti) + r, y = T.py(T.j0 + tj);
+        if (x <= max_x && y <= max_y) {
+            const int c0 = r * R * R + (tj + H) * R + ti + H;
+            const float4 C = sA[c0], Gc = sB[c0];
+            // 3x3 binomial of the variance over the lattice neighbours (offsets +-s, in LDS) and the
+            // per-pixel depth gradient the temporal stage stored
+            float gv = 0.0f;
 #pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
+            for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx)
-                gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c0 + dy * R + dx], gv);
-        const Buf gr = bind(f.grad, f.h);
-        const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
-        const float gx = gxy.x, gy = gxy.y;
-        const float nls = (-LOG2E / p.phi_luma) * rsq_hw(fmaxf(gv, 0.0f) + p.variance_eps);
-        // -log2(e) / (phi_z |g.o| + eps) for the 12 offsets after the centre in row-major
-        // order; the mirrored offset has the bitwise-same denominator.
-        float nrd[12];
+                for (int dx = -1; dx <= 1; ++dx)
+                    gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c0 + dy * R + dx], gv);
+            const Buf gr = bind(f.grad, f.h);
+            const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
+            const float gx = gxy.x, gy = gxy.y;
+            const float nls = (-LOG2E / p.phi_luma) * rsq_hw(fmaxf(gv, 0.0f) + p.variance_eps);
+            // -log2(e) / (phi_z |g.o| + eps) for the 12 offsets after the centre in row-major
+            // order; the mirrored offset has the bitwise-same denominator.
+            float nrd[12];
 #pragma unroll
-        for (int i = 0; i < 12; ++i) {
-            const int dx = (i + 13) % 5 - 2, dy = (i + 13) / 5 - 2;
-            float den = fmaf(p.phi_depth, fabsf(fmaf(gx, (float)(dx * s), gy * (float)(dy * s))), p.depth_eps);
-            nrd[i] = p.phi_depth > 0.0f ? -LOG2E * __builtin_amdgcn_rcpf(den) : 0.0f;
-        }
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sw = 0.0f, sv = 0.0f;
-#pragma unroll
-        for (int dy = -2; dy <= 2; ++dy) {
-#pragma unroll
-            for (int dx = -2; dx <= 2; ++dx) {
-                const int o = (dy + 2) * 5 + dx + 2;  // 0..24, centre 12
-                const int q = c0 + dy * R + dx;
-                const float4 A = sA[q], B = sB[q];
-                float e = log2k(dx + 2, dy + 2);
-                if (o != 12) e = fmaf(fabsf(B.w - Gc.w), nrd[o > 12 ? o - 13 : 11 - o], e);
-                e = fmaf(fabsf(A.w - C.w), nls, e);
-                float d = fmaf(Gc.x, B.x, fmaf(Gc.y, B.y, Gc.z * B.z));
-                e = fmaf(p.phi_normal, __builtin_amdgcn_logf(fmaxf(d, 1e-37f)), e);
-                float w = __builtin_amdgcn_exp2f(e);
-                s0 = fmaf(w, A.x, s0);
-                s1 = fmaf(w, A.y, s1);
-                s2 = fmaf(w, A.z, s2);
-                sw += w;
-                sv = fmaf(w * w, sV[q], sv);
+            for (int i = 0; i < 12; ++i) {
+                const int dx = (i + 13) % 5 - 2, dy = (i + 13) / 5 - 2;
+                float den = fmaf(p.phi_depth, fabsf(fmaf(gx, (float)(dx * s), gy * (float)(dy * s))), p.depth_eps);
+                nrd[i] = p.phi_depth > 0.0f ? -LOG2E * __builtin_amdgcn_rcpf(den) : 0.0f;
             }
-            // The row's sums are complete here. Without this the optimiser sinks the weighted
-            // sums below later rows' exponents and keeps those taps' weights and colours live:
-            // 77 VGPRs instead of 58, 6 waves per SIMD instead of 8, and a quarter slower.
-            asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(sw), "+v"(sv));
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sw = 0.0f, sv = 0.0f;
+#pragma unroll
+            for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+                for (int dx = -2; dx <= 2; ++dx) {
+                    const int o = (dy + 2) * 5 + dx + 2;  // 0..24, centre 12
+                    const int q = c0 + dy * R + dx;
+                    const float4 A = sA[q], B = sB[q];
+                    float e = log2k(dx + 2, dy + 2);
+                    if (o != 12) e = fmaf(fabsf(B.w - Gc.w), nrd[o > 12 ? o - 13 : 11 - o], e);
+                    e = fmaf(fabsf(A.w - C.w), nls, e);
+                    float d = fmaf(Gc.x, B.x, fmaf(Gc.y, B.y, Gc.z * B.z));
+                    e = fmaf(p.phi_normal, __builtin_amdgcn_logf(fmaxf(d, 1e-37f)), e);
+                    float w = __builtin_amdgcn_exp2f(e);
+                    s0 = fmaf(w, A.x, s0);
+                    s1 = fmaf(w, A.y, s1);
+                    s2 = fmaf(w, A.z, s2);
+                    sw += w;
+                    sv = fmaf(w * w, sV[q], sv);
+                }
+                // The row's sums are complete here. Without this the optimiser sinks the weighted
+                // sums below later rows' exponents and keeps those taps' weights and colours live:
+                // 77 VGPRs instead of 58, 6 waves per SIMD instead of 8, and a quarter slower.
+                asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(sw), "+v"(sv));
+            }
+            float4 o;
+            if (sw > 1e-6f) {
+                float inv = rcp_hw(sw);
+                o = make_float4(s0 * inv, s1 * inv, s2 * inv, sqrt_hw(sv) * inv);
+            } else {
+                o = make_float4(C.x, C.y, C.z, sqrt_hw(sV[c0]));
+            }
+            if (k == 0) {
+                const Buf nl = bind(f.nlen, f.h);
+                float n = (float)ld_u8(nl, y * nl.pitch + x);
+                S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, n));
+            }
+            if (k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
+            else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
         }
-        float4 o;
-        if (sw > 1e-6f) {
-            float inv = rcp_hw(sw);
-            o = make_float4(s0 * inv, s1 * inv, s2 * inv, sqrt_hw(sv) * inv);
-        } else {
-            o = make_float4(C.x, C.y, C.z, sqrt_hw(sV[c0]));
-        }
-        if (k == 0) {
-            const Buf nl = bind(f.nlen, f.h);
-            float n = (float)ld_u8(nl, y * nl.pitch + x);
-            S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, n));
-        }
-        if (k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
-        else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
     }
     __syncthreads();  // the next tile of a grid-stride loop reuses the LDS arrays
 }
@@ -639,7 +660,9 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
 // ========================================================================================
 // Kernels
 // ========================================================================================
-enum StageId : int { ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_ATROUS0 = 3, ST_CAN_ATROUS = 4 };
+enum StageId : int {
+    ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_ATROUS0 = 3, ST_CAN_ATROUS = 4, ST_CAN_ATROUS_PAIR = 5
+};
 
 template <class S, int STAGE, bool COMP>
 __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, const Tile& T, int arg)
@@ -647,8 +670,9 @@ __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, 
     if constexpr (STAGE == ST_REF_INIT) ref_init_tile<S>(f, p, T);
     else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_tile<S>(f, p, T, arg);
     else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, COMP>(f, p, T);
-    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, true>(f, p, T, 0);
-    else can_atrous_tile<S, false>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, true, 1>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, false, 1>(f, p, T, arg);
+    else can_atrous_tile<S, false, 2>(f, p, T, arg);
 }
 
 // Frame descriptor of a batch entry into scalar registers (the index is uniform across the block).
@@ -666,7 +690,8 @@ __device__ __forceinline__ FrameDesc load_frame(const FrameDesc* frames, uint32_
 template <int STAGE>
 __device__ __forceinline__ Tile stage_tile(int w, uint32_t log2s, uint32_t t)
 {
-    return decode_tile(w, log2s, t);
+    if constexpr (STAGE == ST_CAN_ATROUS_PAIR) return decode_pair_tile(w, log2s, t);
+    else return decode_tile(w, log2s, t);
 }
 
 // arg: REFERENCE a-trous pass (1..N) or CANONICAL a-trous iteration (0..N-1); log2s: stride.
@@ -796,7 +821,9 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
             if constexpr (CANON) {
                 if (k.stage == 0) persist_tile<S, ST_CAN_TEMPORAL, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
                 else if (k.stage == 1) persist_tile<S, ST_CAN_ATROUS0, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
-                else persist_tile<S, ST_CAN_ATROUS, COMP>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
+                else if (k.stage - 1 < PAIR_LOG2S)
+                    persist_tile<S, ST_CAN_ATROUS, COMP>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
+                else persist_tile<S, ST_CAN_ATROUS_PAIR, COMP>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
             } else {
                 if (k.stage == 0) persist_tile<S, ST_REF_INIT, false>(bt.frames, k.frame, p, 0, k.tile, 0);
                 else persist_tile<S, ST_REF_ATROUS, false>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage);
@@ -830,7 +857,8 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
 template <class S, int STAGE, bool COMP>
 void launch_frame(const FrameDesc& f, const KParams& p, int arg, uint32_t log2s, hipStream_t stream)
 {
-    uint32_t n = tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s);
+    uint32_t n = STAGE == ST_CAN_ATROUS_PAIR ? pair_tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s)
+                                             : tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s);
     hipLaunchKernelGGL((k_frame<S, STAGE, COMP>), dim3(n), dim3(BLOCK), 0, stream, f, p, arg, log2s);
 }
 
@@ -867,7 +895,11 @@ void canonical_frame(const FrameDesc& f, const KParams& p, hipStream_t s, hipEve
     launch_frame<S, ST_CAN_ATROUS0, COMP>(f, p, 0, 0, s);
     mark(ev, 2, s);
     for (int k = 1; k < p.iterations; ++k) {
-        launch_frame<S, ST_CAN_ATROUS, COMP>(f, p, k, (uint32_t)k, s);
+        // A single small frame has few tiles at large strides: halving them (residue pairs)
+        // would leave the device emptier, so pairs are used from 2048 single-residue tiles on.
+        if (k < (int)PAIR_LOG2S || tiles_at((uint32_t)f.w, (uint32_t)f.h, (uint32_t)k) < 2048u)
+            launch_frame<S, ST_CAN_ATROUS, COMP>(f, p, k, (uint32_t)k, s);
+        else launch_frame<S, ST_CAN_ATROUS_PAIR, COMP>(f, p, k, (uint32_t)k, s);
         mark(ev, k + 2, s);
     }
 }
@@ -893,7 +925,8 @@ void canonical_batch(const BatchTable& bt, const KParams& p, const uint32_t* g, 
     launch_batch<S, ST_CAN_ATROUS0, COMP>(bt, p, 0, 0, 1, g, s);
     mark(ev, 2, s);
     for (int k = 1; k < p.iterations; ++k) {
-        launch_batch<S, ST_CAN_ATROUS, COMP>(bt, p, k, (uint32_t)k, k + 1, g, s);
+        if (k < (int)PAIR_LOG2S) launch_batch<S, ST_CAN_ATROUS, COMP>(bt, p, k, (uint32_t)k, k + 1, g, s);
+        else launch_batch<S, ST_CAN_ATROUS_PAIR, COMP>(bt, p, k, (uint32_t)k, k + 1, g, s);
         mark(ev, k + 2, s);
     }
 }
