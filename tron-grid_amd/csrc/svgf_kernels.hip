@@ -562,9 +562,18 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     if constexpr (FIRST) {
         lds_writes_done();
         __syncthreads();  // the staged region and sCount = 0
+        // list the young entries: one LDS atomic per wave (its count), lanes placed by their
+        // rank among the wave's young lanes
 #pragma unroll
-        for (int j = 0; j < NQ; ++j)
-            if (nq[j] < 4u) sList[atomicAdd(&sCount, 1)] = tid + j * BLOCK;
+        for (int j = 0; j < NQ; ++j) {
+            const bool yng = nq[j] < 4u;
+            const uint64_t m = __ballot(yng);
+            int base = 0;
+            if ((tid & 63) == 0 && m) base = atomicAdd(&sCount, (int)__popcll(m));
+            base = __builtin_amdgcn_readfirstlane(base);
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (yng) sList[base + rank] = tid + j * BLOCK;
+        }
         lds_writes_done();
         __syncthreads();
         const int count = sCount;
