@@ -458,40 +458,41 @@ __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KPar
     st_u8(nl, y * nl.pitch + x, (uint32_t)n_new);
 }
 
-// log2 of the depth / normal edge-stop factor between the centre (Gc, depth gradient g) and a
-// tap at pixel offset (ox, oy): -|dz| / (phi_z |g.o| + eps) * log2(e) + phi_n * log2(n.n').
-// Returns -inf when the normal term is zero.
-__device__ __forceinline__ float edge_log2(const KParams& p, float4 Gc, float4 Gq, float gx, float gy, float ox, float oy)
-{
-    float a = 0.0f;
-    if (p.phi_depth > 0.0f) {
-        float den = fmaf(p.phi_depth, fabsf(fmaf(gx, ox, gy * oy)), p.depth_eps);
-        a = -fabsf(Gq.w - Gc.w) * __builtin_amdgcn_rcpf(den) * LOG2E;
-    }
-    if (p.phi_normal > 0.0f) {
-        float d = fmaf(Gc.x, Gq.x, fmaf(Gc.y, Gq.y, Gc.z * Gq.z));
-        a = d > 0.0f ? fmaf(p.phi_normal, __builtin_amdgcn_logf(d), a) : -INFINITY;
-    }
-    return a;
-}
-
 // Spatial variance of a young pixel (history length n < 4): the edge-aware variance of the
-// temporal luminance over its 3x3 window (clamped at the frame border), weighted by the depth /
-// normal edge-stops of the centre. tap(dx, dy) gives the LDS entry holding the window pixel.
+// temporal luminance over its 3x3 window (clamped at the frame border). A tap at offset o is
+// weighted by 2^e with
+//   e = -|z_o - z| / (phi_z |g.o| + eps) log2(e) + phi_n log2(n.n_o)   (n.n_o <= 0: weight 0)
+// where g is the centre's depth gradient; each term only when its phi is positive. The
+// denominator of o and -o is the same float, so five reciprocals serve the nine taps. The
+// code is branch-free per tap (selects), so the taps' LDS reads overlap. tap(dx, dy) gives
+// the LDS entry holding the window pixel.
 template <class Tap>
 __device__ __forceinline__ float young_variance(const KParams& p, const float4* sA, const float4* sB, float4 G,
                                                 float gx, float gy, Tap tap)
 {
+    // o = (0,0), (1,0), (0,1), (1,1), (-1,1); a mirrored offset uses its partner's entry
+    float nrd[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const float ox = i == 1 || i == 3 ? 1.0f : (i == 4 ? -1.0f : 0.0f), oy = i >= 2 ? 1.0f : 0.0f;
+        nrd[i] = __builtin_amdgcn_rcpf(fmaf(p.phi_depth, fabsf(fmaf(gx, ox, gy * oy)), p.depth_eps));
+    }
+    const bool use_z = p.phi_depth > 0.0f, use_n = p.phi_normal > 0.0f;
     float tw[9], tl[9], sw = 0.0f, s1 = 0.0f;
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy) {
 #pragma unroll
         for (int dx = -1; dx <= 1; ++dx) {
             const int t = (dy + 1) * 3 + dx + 1;
-            const int q = tap(dx, dy);
-            float w = __builtin_amdgcn_exp2f(edge_log2(p, G, sB[q], gx, gy, (float)dx, (float)dy));
+            const int mi = dy == 0 ? (dx == 0 ? 0 : 1) : (dx == 0 ? 2 : (dx == dy ? 3 : 4));
+            const float4 Gq = sB[tap(dx, dy)];
+            const float a = use_z ? -fabsf(Gq.w - G.w) * nrd[mi] * LOG2E : 0.0f;
+            const float d = fmaf(G.x, Gq.x, fmaf(G.y, Gq.y, G.z * Gq.z));
+            const float lg = __builtin_amdgcn_logf(d);
+            const float e = use_n ? (d > 0.0f ? fmaf(p.phi_normal, lg, a) : -INFINITY) : a;
+            const float w = __builtin_amdgcn_exp2f(e);
             tw[t] = w;
-            tl[t] = sA[q].w;
+            tl[t] = sA[tap(dx, dy)].w;
             sw += w;
             s1 = fmaf(w, tl[t], s1);
         }
