@@ -161,10 +161,12 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
     for i, (w, h) in enumerate(sizes):
         fr = []
         for f in range(2):
+            # ticks alternate frames 0 and 1: frame 1 follows frame 0 (motion +pan) and frame 0
+            # follows frame 1 (motion -pan), so the history stays valid across ticks
             d = S.synth_frame(w, h, seed=77 * i + f, stream=i, frame=f, pan_px=0.5, disocclusion=0.05)
             fr.append({"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
                        "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
-                       "motion": torch.from_numpy(d["motion"]).to(dev)})
+                       "motion": torch.from_numpy(d["motion"] * (1.0 if f else -1.0)).to(dev)})
         ins.append(fr)
     outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes]
     from svgf_amd.scenes import composite_scene
@@ -450,6 +452,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--frames", type=int, default=64, help="frames per batch")
     ap.add_argument("--distinct", type=int, default=16, help="distinct input frames per stream (cycled)")
+    ap.add_argument("--sequence", choices=["pingpong", "cut"], default="pingpong",
+                    help="how the distinct frames repeat: forward and back (continuous), or in a loop (history "
+                         "rejected at each restart)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--agents", type=int, default=256, help="agent streams of the agent-frame tick (0: skip)")
     ap.add_argument("--agent-ticks", type=int, default=20)
@@ -521,7 +526,17 @@ def main():
     maxw = max(w for w, _ in shapes)
     maxh = max(h for _, h in shapes)
     ND = args.distinct
-    assert ND % 2 == 0, "distinct frames must be even (ping-pong parity)"
+    assert ND % 2 == 0 and ND >= 2, "distinct frames must be even (ping-pong parity)"
+    # Step k of a stream shows frame seq[k % P] of its ND distinct frames. The default runs
+    # them forward and back (0, 1, .., ND-1, ND-2, .., 1) with the motion negated on the way
+    # back, so every step continues the previous one (a continuous pan, as the generator
+    # describes); "--sequence cut" restarts at frame 0 after ND-1, which rejects the whole
+    # history there (a camera cut every ND frames).
+    frames_seq = list(range(ND)) + (list(range(ND - 2, 0, -1)) if args.sequence == "pingpong" else [])
+    P = len(frames_seq)
+    # frame f shows the world at u = x + f * pan, so from the previous step's frame fp a pixel
+    # moved by (f - fp) * pan: motion = +pan forward, -pan backward (a restart keeps +pan)
+    seq = [(f, -1.0 if frames_seq[(q - 1) % P] == f + 1 else 1.0) for q, f in enumerate(frames_seq)]
 
     progress("inputs")
     # ---- inputs: ND frames per stream, generated on the host, resident in HBM ----
@@ -531,6 +546,7 @@ def main():
     rad32 = [[None] * ND for _ in shapes]
     gbuf = [[None] * ND for _ in shapes]
     mvec = [[None] * ND for _ in shapes]
+    mneg = [[None] * ND for _ in shapes]  # motion of the backward half of the ping-pong sequence
     for i, (w, h) in enumerate(shapes):
         for f in range(ND):
             d = S.synth_frame(w, h, seed=1000 * i + f, stream=i, frame=f, pan_px=0.5, disocclusion=0.05)
@@ -538,6 +554,7 @@ def main():
             rad32[i][f] = torch.from_numpy(d["radiance"]).to(dev)
             gbuf[i][f] = torch.from_numpy(d["gbuffer"]).to(dev)
             mvec[i][f] = torch.from_numpy(d["motion"]).to(dev)
+            mneg[i][f] = -mvec[i][f]
             if f < 2:
                 host[i][f] = d
     torch.cuda.synchronize()
@@ -548,14 +565,14 @@ def main():
     p16 = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
     out16 = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in shapes]
     reqs = []
-    for f in range(ND):
+    for f, sgn in seq:
         arr = (S.Request * args.frames)()
         for i, (w, h) in enumerate(shapes):
             r = arr[i]
             r.stream_id = i  # every batched context below opens its streams in this order
             r.radiance = S.plane_of(rad16[i][f], S.FMT_RGBA16F)
             r.gbuffer = S.plane_of(gbuf[i][f], S.FMT_RGBA32F)
-            r.motion = S.plane_of(mvec[i][f], S.FMT_RG32F)
+            r.motion = S.plane_of(mvec[i][f] if sgn > 0 else mneg[i][f], S.FMT_RG32F)
             r.denoised = S.plane_of(out16[i], S.FMT_RGBA16F)
         reqs.append(arr)
     L = S.lib()
@@ -570,7 +587,7 @@ def main():
         return ctx
 
     def batch_step(ctx, k):
-        S.check(L.svgf_denoise_batch(ctx.handle, reqs[k % ND], args.frames, sh), "svgf_denoise_batch")
+        S.check(L.svgf_denoise_batch(ctx.handle, reqs[k % P], args.frames, sh), "svgf_denoise_batch")
 
     def time_batched(flags):
         ctx = batched_context(flags)
@@ -615,25 +632,26 @@ def main():
     mom = [[torch.zeros((h, w, 2), device=dev) for _ in range(2)] for w, h in shapes]
     out32 = [torch.empty((h, w, 4), device=dev) for w, h in shapes]
     frames32 = []  # [f][i] prebuilt descriptors; frame f reads parity (f+1)%2, writes f%2
-    for f in range(ND):
+    for q, (f, sgn) in enumerate(seq):
+        fp = seq[(q - 1) % P][0]  # frame of the previous step
         row = []
         for i, (w, h) in enumerate(shapes):
             fr = S.Frame()
             fr.width, fr.height, fr.history_valid = w, h, 1
             fr.radiance = S.plane_of(rad32[i][f], S.FMT_RGBA32F)
             fr.gbuffer = S.plane_of(gbuf[i][f], S.FMT_RGBA32F)
-            fr.gbuffer_prev = S.plane_of(gbuf[i][(f - 1) % ND], S.FMT_RGBA32F)
-            fr.motion = S.plane_of(mvec[i][f], S.FMT_RG32F)
-            fr.history_prev = S.plane_of(hist[i][(f + 1) % 2], S.FMT_RGBA32F)
-            fr.history_curr = S.plane_of(hist[i][f % 2], S.FMT_RGBA32F)
-            fr.moments_prev = S.plane_of(mom[i][(f + 1) % 2], S.FMT_RG32F)
-            fr.moments_curr = S.plane_of(mom[i][f % 2], S.FMT_RG32F)
+            fr.gbuffer_prev = S.plane_of(gbuf[i][fp], S.FMT_RGBA32F)
+            fr.motion = S.plane_of(mvec[i][f] if sgn > 0 else mneg[i][f], S.FMT_RG32F)
+            fr.history_prev = S.plane_of(hist[i][(q + 1) % 2], S.FMT_RGBA32F)
+            fr.history_curr = S.plane_of(hist[i][q % 2], S.FMT_RGBA32F)
+            fr.moments_prev = S.plane_of(mom[i][(q + 1) % 2], S.FMT_RG32F)
+            fr.moments_curr = S.plane_of(mom[i][q % 2], S.FMT_RG32F)
             fr.denoised = S.plane_of(out32[i], S.FMT_RGBA32F)
             row.append(fr)
         frames32.append(row)
 
     def plain_step(ctx, k):
-        for fr in frames32[k % ND]:
+        for fr in frames32[k % P]:
             S.check(L.svgf_denoise(ctx.handle, ctypes.byref(fr), sh), "svgf_denoise")
 
     with torch.cuda.stream(stream):
@@ -726,10 +744,12 @@ def main():
         for c, (w, h) in enumerate(classes):
             ins = []
             for f in range(2):
+                # requests of a stream alternate frames 0 and 1 (motion -pan on frame 0: it
+                # follows frame 1), so a stream's history stays valid
                 d = S.synth_frame(w, h, seed=7000 + 10 * c + f, stream=10000 + c, frame=f, pan_px=0.5, disocclusion=0.05)
                 ins.append({"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
                             "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
-                            "motion": torch.from_numpy(d["motion"]).to(dev)})
+                            "motion": torch.from_numpy(d["motion"] * (1.0 if f else -1.0)).to(dev)})
             class_inputs.append(ins)
         torch.cuda.synchronize()
         budget = 64 << 30
@@ -811,7 +831,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp16-storage/fp32-math",
-        "data": f"synthetic seeded G-buffers ({ND} distinct frames per stream, cycled)",
+        "data": (f"synthetic seeded G-buffers, {ND} distinct frames per stream run "
+                 + ("forward and back (continuous pan)" if args.sequence == "pingpong" else "in a loop (a cut every "
+                    f"{ND} frames)")),
         "config": {
             "workload": ("batched canonical SVGF, 64 agent frames per persistent launch (ragged tile tickets), hipGraph"
                          if head == "persistent" else
