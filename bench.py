@@ -815,7 +815,7 @@ def main():
         tfile = ROOT / "profiles" / "traffic.txt"
         if tfile.exists() and args.frames == 64:
             for ln in tfile.read_text().splitlines():
-                if ln.startswith("atrous_batched_fp16_mix64_bytes_per_launch:"):
+                if ln.startswith("atrous_pair_batched_fp16_mix64_bytes_per_launch:"):
                     traffic = float(ln.split(":", 1)[1])
 
     ms_per_step = t_batched / args.steps * 1e3
