@@ -110,6 +110,27 @@ def test_canonical_fp32_sequence(S, oracle_lib):
         assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), "history length differs"
 
 
+@pytest.mark.parametrize("phis", [(4.0, 0.0, 0.0), (4.0, 8.0, 0.5), (2.0, 128.0, 0.0), (10.0, 0.0, 3.0)],
+                         ids=["luma_only", "weak_normal", "no_depth", "no_normal"])
+def test_canonical_edge_stop_settings(S, oracle_lib, phis):
+    """Edge-stop weights other than the defaults, including each of the normal and depth terms
+    switched off (phi 0): FP32 GPU vs FP64 oracle over 3 frames."""
+    O = oracle_lib
+    w, h = 120, 80
+    inputs = seq_inputs(w, h, 3, pan_px=1.37, disocclusion=0.1)
+    op = O.canonical_params(5)
+    gp = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP32)
+    for p in (op, gp):
+        p.phi_luma, p.phi_normal, p.phi_depth = phis
+    ref = oracle_canonical_sequence(inputs, op)
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP32, params=gp, max_width=w, max_height=h) as ctx:
+        got = gpu_canonical_sequence(ctx, inputs, torch.float32)
+    for f, (g, r) in enumerate(zip(got, ref)):
+        dm, dmean = rel_err(g["denoised"][..., :3], r["denoised"][..., :3])
+        print(f"{phis} frame {f}: denoised max {dm:.3e} mean {dmean:.3e}")
+        assert dm <= 2e-4 and dmean <= 1e-6, f
+
+
 def test_canonical_fp16_compensated_sequence(S, oracle_lib):
     """Precision form: FP16 storage, compensated moments, lognormal noise, vs FP64 oracle."""
     O = oracle_lib

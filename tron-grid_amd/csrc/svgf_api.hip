@@ -8,6 +8,7 @@
 #include "svgf_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -126,6 +127,10 @@ KParams to_kparams(const svgf_params& p)
     k.history_cap = (int32_t)p.history_cap;
     k.iterations = (int32_t)p.iterations;
     k.compensated = (p.flags & SVGF_FLAG_COMPENSATED_MOMENTS) ? 1 : 0;
+    constexpr float LOG2E = 1.44269504f, LN2 = 0.693147181f;
+    k.luma_k = p.phi_luma > 0.0f ? -LOG2E / p.phi_luma : 0.0f;
+    k.depth_c = LN2 * p.phi_depth;
+    k.depth_e = p.phi_depth > 0.0f ? -LN2 * p.depth_eps : -INFINITY;
     return k;
 }
 

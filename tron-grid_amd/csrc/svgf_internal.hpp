@@ -47,6 +47,13 @@ struct KParams {
     int32_t history_cap;
     int32_t iterations;
     int32_t compensated;
+    // Canonical a-trous edge-stop constants, folded on the host so the exponent of a tap is
+    // two fused multiply-adds (the filter works in base 2):
+    //   luma_k  = -log2(e) / phi_luma
+    //   depth_c = ln(2) phi_depth,  depth_e = -ln(2) depth_eps (-inf when phi_depth is 0)
+    // so that -log2(e) / (phi_depth |t| + depth_eps) = 1 / (depth_e - depth_c |t|), which is
+    // -0 (depth term off) when phi_depth is 0.
+    float luma_k, depth_c, depth_e;
 };
 
 // Batched form: the device-side table of a batch (header and starts rows as in

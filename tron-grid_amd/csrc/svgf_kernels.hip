@@ -551,14 +551,31 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             nq[j] = e < LREG * LREG ? ld_u8(nl, qy * nl.pitch + qx) : 4u;
         }
     }
-    for (int e2 = tid; e2 < NR * R * R; e2 += BLOCK) {
-        const int r = e2 % NR, e = e2 / NR;  // neighbouring lanes: the two residues of one entry
-        int li = e % R, lj = e / R;
-        int qx = clampi(T.px(T.i0 - H + li) + r, 0, max_x), qy = clampi(T.py(T.j0 - H + lj), 0, max_y);
-        float4 v = WavS::ld4(src, qx, qy);
-        sA[r * R * R + e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
-        sB[r * R * R + e] = ldg4(gb, qx, qy);
-        sV[r * R * R + e] = v.w * v.w;
+    {
+        // every load of the region is issued before the first LDS write (one memory round trip);
+        // the last pass's surplus lanes load the final entry again and store nothing
+        constexpr int NE = NR * R * R, NIT = (NE + BLOCK - 1) / BLOCK;
+        float4 va[NIT], vb[NIT];
+#pragma unroll
+        for (int j = 0; j < NIT; ++j) {
+            const int e2 = min(tid + j * BLOCK, NE - 1);
+            const int r = e2 % NR, e = e2 / NR;  // neighbouring lanes: the two residues of one entry
+            const int li = e % R, lj = e / R;
+            const int qx = clampi(T.px(T.i0 - H + li) + r, 0, max_x), qy = clampi(T.py(T.j0 - H + lj), 0, max_y);
+            va[j] = WavS::ld4(src, qx, qy);
+            vb[j] = ldg4(gb, qx, qy);
+        }
+#pragma unroll
+        for (int j = 0; j < NIT; ++j) {
+            const int e2 = tid + j * BLOCK;
+            if (j + 1 < NIT || e2 < NE) {
+                const int r = e2 % NR, e = e2 / NR;
+                const float4 v = va[j];
+                sA[r * R * R + e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
+                sB[r * R * R + e] = vb[j];
+                sV[r * R * R + e] = v.w * v.w;
+            }
+        }
     }
     if constexpr (FIRST) {
         lds_writes_done();
@@ -602,6 +619,9 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         if (x <= max_x && y <= max_y) {
             const int c0 = r * R * R + (tj + H) * R + ti + H;
             const float4 C = sA[c0], Gc = sB[c0];
+            const bool nrm_on = p.phi_normal > 0.0f;
+            const float3 N = nrm_on ? make_float3(Gc.x, Gc.y, Gc.z) : make_float3(0.0f, 0.0f, 0.0f);
+            const float nb = nrm_on ? 0.0f : 1.0f;
             // 3x3 binomial of the variance over the lattice neighbours (offsets +-s, in LDS) and the
             // per-pixel depth gradient the temporal stage stored
             float gv = 0.0f;
@@ -613,16 +633,20 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             const Buf gr = bind(f.grad, f.h);
             const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
             const float gx = gxy.x, gy = gxy.y;
-            const float nls = (-LOG2E / p.phi_luma) * rsq_hw(fmaxf(gv, 0.0f) + p.variance_eps);
-            // -log2(e) / (phi_z |g.o| + eps) for the 12 offsets after the centre in row-major
-            // order; the mirrored offset has the bitwise-same denominator.
+            const float nls = p.luma_k * rsq_hw(fmaxf(gv, 0.0f) + p.variance_eps);
+            // -log2(e) / (phi_z |g.o| + eps) = 1 / (depth_e - |dx U + dy V|) with (U, V) = ln(2)
+            // phi_z s g, for the 12 offsets after the centre in row-major order, (1,0) (2,0)
+            // (-2,1) (-1,1) (0,1) (1,1) (2,1) (-2,2) (-1,2) (0,2) (1,2) (2,2); the mirrored
+            // offset has the bitwise-same denominator. |dx U + dy V| = tk |tx| (the factors 2
+            // are exact). All -0 when phi_z is 0.
+            const float cs = p.depth_c * (float)s, U = gx * cs, V = gy * cs;
+            const float VmU = V - U, UpV = U + V;
+            const float tx[12] = {U,   U, fmaf(-2.0f, U, V), VmU, V, UpV, fmaf(2.0f, U, V),
+                                  VmU, fmaf(2.0f, V, -U),    V,   fmaf(2.0f, V, U), UpV};
+            constexpr float tk[12] = {1.0f, 2.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 2.0f, 1.0f, 2.0f, 1.0f, 2.0f};
             float nrd[12];
 #pragma unroll
-            for (int i = 0; i < 12; ++i) {
-                const int dx = (i + 13) % 5 - 2, dy = (i + 13) / 5 - 2;
-                float den = fmaf(p.phi_depth, fabsf(fmaf(gx, (float)(dx * s), gy * (float)(dy * s))), p.depth_eps);
-                nrd[i] = p.phi_depth > 0.0f ? -LOG2E * __builtin_amdgcn_rcpf(den) : 0.0f;
-            }
+            for (int i = 0; i < 12; ++i) nrd[i] = __builtin_amdgcn_rcpf(fmaf(-tk[i], fabsf(tx[i]), p.depth_e));
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sw = 0.0f, sv = 0.0f;
 #pragma unroll
             for (int dy = -2; dy <= 2; ++dy) {
@@ -634,8 +658,11 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
                     float e = log2k(dx + 2, dy + 2);
                     if (o != 12) e = fmaf(fabsf(B.w - Gc.w), nrd[o > 12 ? o - 13 : 11 - o], e);
                     e = fmaf(fabsf(A.w - C.w), nls, e);
-                    float d = fmaf(Gc.x, B.x, fmaf(Gc.y, B.y, Gc.z * B.z));
-                    e = fmaf(p.phi_normal, __builtin_amdgcn_logf(fmaxf(d, 1e-37f)), e);
+                    // max(0, n.n_q)^phi_n as phi_n log2(d) in the exponent, d clamped to [0, 1]
+                    // by the dot product's clamp bit: d <= 0 gives log2 = -inf, weight 0 (unit
+                    // normals exceed 1 only by rounding). With phi_n = 0, d = 1 (N = 0, nb = 1).
+                    const float d = __builtin_amdgcn_fmed3f(fmaf(N.x, B.x, fmaf(N.y, B.y, fmaf(N.z, B.z, nb))), 0.0f, 1.0f);
+                    e = fmaf(p.phi_normal, __builtin_amdgcn_logf(d), e);
                     float w = __builtin_amdgcn_exp2f(e);
                     s0 = fmaf(w, A.x, s0);
                     s1 = fmaf(w, A.y, s1);
