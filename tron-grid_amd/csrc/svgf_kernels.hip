@@ -511,6 +511,83 @@ __device__ __forceinline__ float young_variance(const KParams& p, const float4* 
     return var;
 }
 
+// Per-pixel terms of an a-trous tap weight: luminance and depth of the centre, the luminance
+// scale -log2(e) / sigma_l, the normal (zero and nb = 1 when the normal stop is off, so every
+// tap's clamped dot product is 1) and the depth scales of the 12 offsets after the centre.
+struct Centre {
+    float lc, zc, nls, nx, ny, nz, nb;
+    float nrd[12];
+};
+
+// C: centre (colour, luminance), Gc: centre (normal, depth), gv: prefiltered variance,
+// (gx, gy): depth gradient, s: stride.
+__device__ __forceinline__ Centre centre_of(const KParams& p, float4 C, float4 Gc, float gv, float gx, float gy, int s)
+{
+    Centre c;
+    c.lc = C.w;
+    c.zc = Gc.w;
+    c.nls = p.luma_k * rsq_hw(fmaxf(gv, 0.0f) + p.variance_eps);
+    const bool nrm_on = p.phi_normal > 0.0f;
+    c.nx = nrm_on ? Gc.x : 0.0f;
+    c.ny = nrm_on ? Gc.y : 0.0f;
+    c.nz = nrm_on ? Gc.z : 0.0f;
+    c.nb = nrm_on ? 0.0f : 1.0f;
+    // -log2(e) / (phi_z |g.o| + eps) = 1 / (depth_e - |dx U + dy V|) with (U, V) = ln(2) phi_z s g,
+    // for the 12 offsets after the centre in row-major order, (1,0) (2,0) (-2,1) (-1,1) (0,1)
+    // (1,1) (2,1) (-2,2) (-1,2) (0,2) (1,2) (2,2); the mirrored offset has the bitwise-same
+    // denominator. |dx U + dy V| = tk |tx| (the factors 2 are exact). All -0 when phi_z is 0.
+    const float cs = p.depth_c * (float)s, U = gx * cs, V = gy * cs;
+    const float VmU = V - U, UpV = U + V;
+    const float tx[12] = {U,   U, fmaf(-2.0f, U, V), VmU, V, UpV, fmaf(2.0f, U, V),
+                          VmU, fmaf(2.0f, V, -U),    V,   fmaf(2.0f, V, U), UpV};
+    constexpr float tk[12] = {1.0f, 2.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 2.0f, 1.0f, 2.0f, 1.0f, 2.0f};
+#pragma unroll
+    for (int i = 0; i < 12; ++i) c.nrd[i] = __builtin_amdgcn_rcpf(fmaf(-tk[i], fabsf(tx[i]), p.depth_e));
+    return c;
+}
+
+// Weight of the tap at offset (dx, dy) (compile-time after unrolling) with entry colour A and
+// (normal, depth) B: K[dx] K[dy] 2^e with e the sum of the depth, luminance and normal terms.
+__device__ __forceinline__ float tap_weight(const KParams& p, const Centre& c, int dx, int dy, float4 A, float4 B)
+{
+    const int o = (dy + 2) * 5 + dx + 2;  // 0..24, centre 12
+    float e = log2k(dx + 2, dy + 2);
+    if (o != 12) e = fmaf(fabsf(B.w - c.zc), c.nrd[o > 12 ? o - 13 : 11 - o], e);
+    e = fmaf(fabsf(A.w - c.lc), c.nls, e);
+    // max(0, n.n_q)^phi_n as phi_n log2(d) in the exponent, d clamped to [0, 1] by the dot
+    // product's clamp bit: d <= 0 gives log2 = -inf, weight 0 (unit normals exceed 1 only by
+    // rounding)
+    const float d = __builtin_amdgcn_fmed3f(fmaf(c.nx, B.x, fmaf(c.ny, B.y, fmaf(c.nz, B.z, c.nb))), 0.0f, 1.0f);
+    e = fmaf(p.phi_normal, __builtin_amdgcn_logf(d), e);
+    return __builtin_amdgcn_exp2f(e);
+}
+
+// Weighted sums of a pixel: colour (3), weight, squared-weight variance.
+struct Sums {
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sw = 0.0f, sv = 0.0f;
+    __device__ __forceinline__ void add(float w, float4 A, float V)
+    {
+        s0 = fmaf(w, A.x, s0);
+        s1 = fmaf(w, A.y, s1);
+        s2 = fmaf(w, A.z, s2);
+        sw += w;
+        sv = fmaf(w * w, V, sv);
+    }
+    // The row's sums are complete here. Without this the optimiser sinks the weighted sums
+    // below later rows' exponents and keeps those taps' weights and colours live (77 VGPRs
+    // instead of 58 in the one-pixel filter, 6 waves per SIMD instead of 8, a quarter slower).
+    __device__ __forceinline__ void pin() { asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(sw), "+v"(sv)); }
+    // (colour, sigma) of the filtered pixel; the centre's own when the weights vanish
+    __device__ __forceinline__ float4 result(float4 C, float Vc) const
+    {
+        if (sw > 1e-6f) {
+            const float inv = rcp_hw(sw);
+            return make_float4(s0 * inv, s1 * inv, s2 * inv, sqrt_hw(sv) * inv);
+        }
+        return make_float4(C.x, C.y, C.z, sqrt_hw(Vc));
+    }
+};
+
 // A-trous iteration k (stride s = 2^k) on one lattice tile: luminance, depth and normal
 // edge-stops, variance filtered by squared weights. Iteration 0 also writes the colour
 // history, the last one the output.
@@ -520,8 +597,8 @@ __device__ __forceinline__ float young_variance(const KParams& p, const float4* 
 // this iteration stages a 22x22 region; the young entries of the tap region are listed and
 // the block's threads compute their windows from LDS before the filter runs.
 // NR = 2: the tile holds residues (rx, ry) and (rx + 1, ry) (strides >= 8): the region of each
-// is staged (the pair's two pixels of an entry are loaded by neighbouring lanes, one cache line)
-// and every thread filters its lattice point in both.
+// is staged (the pair's two pixels of an entry are loaded by neighbouring lanes, one cache line);
+// each half of the block filters one residue, a thread two vertically adjacent lattice points.
 template <class S, bool FIRST, int NR>
 __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
 {
@@ -612,83 +689,77 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         lds_writes_done();
     }
     __syncthreads();
-    const int ti = tid & (LT - 1), tj = tid >> 4;
-#pragma unroll 1
-    for (int r = 0; r < NR; ++r) {
-        const int x = T.px(T.i0 + ti) + r, y = T.py(T.j0 + tj);
+    // 3x3 binomial of the variance over the lattice neighbours (offsets +-s, in LDS)
+    auto prefilter = [&](int c) {
+        float gv = 0.0f;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx)
+                gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c + dy * R + dx], gv);
+        return gv;
+    };
+    const Buf gr = bind(f.grad, f.h);
+    auto store = [&](int x, int y, float4 o) {
+        if (k == 0) {
+            const Buf nl = bind(f.nlen, f.h);
+            const float n = (float)ld_u8(nl, y * nl.pitch + x);
+            S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, n));
+        }
+        if (k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
+        else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
+    };
+    if constexpr (NR == 1) {
+        const int ti = tid & (LT - 1), tj = tid >> 4;
+        const int x = T.px(T.i0 + ti), y = T.py(T.j0 + tj);
         if (x <= max_x && y <= max_y) {
-            const int c0 = r * R * R + (tj + H) * R + ti + H;
-            const float4 C = sA[c0], Gc = sB[c0];
-            const bool nrm_on = p.phi_normal > 0.0f;
-            const float3 N = nrm_on ? make_float3(Gc.x, Gc.y, Gc.z) : make_float3(0.0f, 0.0f, 0.0f);
-            const float nb = nrm_on ? 0.0f : 1.0f;
-            // 3x3 binomial of the variance over the lattice neighbours (offsets +-s, in LDS) and the
-            // per-pixel depth gradient the temporal stage stored
-            float gv = 0.0f;
-#pragma unroll
-            for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-                for (int dx = -1; dx <= 1; ++dx)
-                    gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c0 + dy * R + dx], gv);
-            const Buf gr = bind(f.grad, f.h);
-            const float2 gxy = ld_f2(gr, y * gr.pitch + x * 8);
-            const float gx = gxy.x, gy = gxy.y;
-            const float nls = p.luma_k * rsq_hw(fmaxf(gv, 0.0f) + p.variance_eps);
-            // -log2(e) / (phi_z |g.o| + eps) = 1 / (depth_e - |dx U + dy V|) with (U, V) = ln(2)
-            // phi_z s g, for the 12 offsets after the centre in row-major order, (1,0) (2,0)
-            // (-2,1) (-1,1) (0,1) (1,1) (2,1) (-2,2) (-1,2) (0,2) (1,2) (2,2); the mirrored
-            // offset has the bitwise-same denominator. |dx U + dy V| = tk |tx| (the factors 2
-            // are exact). All -0 when phi_z is 0.
-            const float cs = p.depth_c * (float)s, U = gx * cs, V = gy * cs;
-            const float VmU = V - U, UpV = U + V;
-            const float tx[12] = {U,   U, fmaf(-2.0f, U, V), VmU, V, UpV, fmaf(2.0f, U, V),
-                                  VmU, fmaf(2.0f, V, -U),    V,   fmaf(2.0f, V, U), UpV};
-            constexpr float tk[12] = {1.0f, 2.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 2.0f, 1.0f, 2.0f, 1.0f, 2.0f};
-            float nrd[12];
-#pragma unroll
-            for (int i = 0; i < 12; ++i) nrd[i] = __builtin_amdgcn_rcpf(fmaf(-tk[i], fabsf(tx[i]), p.depth_e));
-            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sw = 0.0f, sv = 0.0f;
+            const int c0 = (tj + H) * R + ti + H;
+            const float4 C = sA[c0];
+            // the per-pixel depth gradient the temporal stage stored
+            const float2 g = ld_f2(gr, y * gr.pitch + x * 8);
+            const Centre c = centre_of(p, C, sB[c0], prefilter(c0), g.x, g.y, s);
+            Sums a;
 #pragma unroll
             for (int dy = -2; dy <= 2; ++dy) {
 #pragma unroll
                 for (int dx = -2; dx <= 2; ++dx) {
-                    const int o = (dy + 2) * 5 + dx + 2;  // 0..24, centre 12
                     const int q = c0 + dy * R + dx;
-                    const float4 A = sA[q], B = sB[q];
-                    float e = log2k(dx + 2, dy + 2);
-                    if (o != 12) e = fmaf(fabsf(B.w - Gc.w), nrd[o > 12 ? o - 13 : 11 - o], e);
-                    e = fmaf(fabsf(A.w - C.w), nls, e);
-                    // max(0, n.n_q)^phi_n as phi_n log2(d) in the exponent, d clamped to [0, 1]
-                    // by the dot product's clamp bit: d <= 0 gives log2 = -inf, weight 0 (unit
-                    // normals exceed 1 only by rounding). With phi_n = 0, d = 1 (N = 0, nb = 1).
-                    const float d = __builtin_amdgcn_fmed3f(fmaf(N.x, B.x, fmaf(N.y, B.y, fmaf(N.z, B.z, nb))), 0.0f, 1.0f);
-                    e = fmaf(p.phi_normal, __builtin_amdgcn_logf(d), e);
-                    float w = __builtin_amdgcn_exp2f(e);
-                    s0 = fmaf(w, A.x, s0);
-                    s1 = fmaf(w, A.y, s1);
-                    s2 = fmaf(w, A.z, s2);
-                    sw += w;
-                    sv = fmaf(w * w, sV[q], sv);
+                    const float4 A = sA[q];
+                    a.add(tap_weight(p, c, dx, dy, A, sB[q]), A, sV[q]);
                 }
-                // The row's sums are complete here. Without this the optimiser sinks the weighted
-                // sums below later rows' exponents and keeps those taps' weights and colours live:
-                // 77 VGPRs instead of 58, 6 waves per SIMD instead of 8, and a quarter slower.
-                asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(sw), "+v"(sv));
+                a.pin();
             }
-            float4 o;
-            if (sw > 1e-6f) {
-                float inv = rcp_hw(sw);
-                o = make_float4(s0 * inv, s1 * inv, s2 * inv, sqrt_hw(sv) * inv);
-            } else {
-                o = make_float4(C.x, C.y, C.z, sqrt_hw(sV[c0]));
+            store(x, y, a.result(C, sV[c0]));
+        }
+    } else {
+        // residue r = tid / 128; the thread filters the vertically adjacent lattice points
+        // (ti, 2u) and (ti, 2u + 1), whose 5x5 taps share 4 of their 5 rows: 6 rows of LDS
+        // entries serve both.
+        const int r = tid >> 7, l = tid & 127, ti = l & (LT - 1), u = l >> 4;
+        const int x = T.px(T.i0 + ti) + r, y0 = T.py(T.j0 + 2 * u), y1 = y0 + s;
+        if (x <= max_x && y0 <= max_y) {
+            const int c0 = r * R * R + (2 * u + H) * R + ti + H, c1 = c0 + R;
+            const float4 C0 = sA[c0], C1 = sA[c1];
+            const float2 g0 = ld_f2(gr, y0 * gr.pitch + x * 8);
+            const float2 g1 = ld_f2(gr, min(y1, max_y) * gr.pitch + x * 8);  // unused past the frame
+            const Centre e0 = centre_of(p, C0, sB[c0], prefilter(c0), g0.x, g0.y, s);
+            const Centre e1 = centre_of(p, C1, sB[c1], prefilter(c1), g1.x, g1.y, s);
+            Sums a0, a1;
+#pragma unroll
+            for (int j = -2; j <= 3; ++j) {
+#pragma unroll
+                for (int dx = -2; dx <= 2; ++dx) {
+                    const int q = c0 + j * R + dx;
+                    const float4 A = sA[q], B = sB[q];
+                    const float V = sV[q];
+                    if (j <= 2) a0.add(tap_weight(p, e0, dx, j, A, B), A, V);
+                    if (j >= -1) a1.add(tap_weight(p, e1, dx, j - 1, A, B), A, V);
+                }
+                a0.pin();
+                a1.pin();
             }
-            if (k == 0) {
-                const Buf nl = bind(f.nlen, f.h);
-                float n = (float)ld_u8(nl, y * nl.pitch + x);
-                S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, n));
-            }
-            if (k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
-            else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
+            store(x, y0, a0.result(C0, sV[c0]));
+            if (y1 <= max_y) store(x, y1, a1.result(C1, sV[c1]));
         }
     }
     __syncthreads();  // the next tile of a grid-stride loop reuses the LDS arrays
