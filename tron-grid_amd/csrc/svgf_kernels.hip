@@ -211,7 +211,7 @@ __device__ __forceinline__ Tile decode_tile(int w, uint32_t log2s, uint32_t t)
     return Tile{(int)log2s, (int)(res & (s - 1)), (int)(res >> log2s), (int)(tx * LT), (int)(ty * LT)};
 }
 
-// Tile t of a residue-pair stage (stride s >= 4): residues (rx, ry) and (rx + 1, ry), rx even,
+// Tile t of a residue-pair stage (stride s >= 2): residues (rx, ry) and (rx + 1, ry), rx even,
 // ordered as decode_tile orders single residues.
 __device__ __forceinline__ Tile decode_pair_tile(int w, uint32_t log2s, uint32_t t)
 {
@@ -596,7 +596,7 @@ struct Sums {
 // entry of the 20x20 tap region, and a young entry's 3x3 window reaches one pixel further, so
 // this iteration stages a 22x22 region; the young entries of the tap region are listed and
 // the block's threads compute their windows from LDS before the filter runs.
-// NR = 2: the tile holds residues (rx, ry) and (rx + 1, ry) (strides >= 8): the region of each
+// NR = 2: the tile holds residues (rx, ry) and (rx + 1, ry) (strides >= 2): the region of each
 // is staged (the pair's two pixels of an entry are loaded by neighbouring lanes, one cache line);
 // each half of the block filters one residue, a thread two vertically adjacent lattice points.
 template <class S, bool FIRST, int NR>

@@ -37,12 +37,13 @@ SVGF_HD inline uint32_t tiles_at(uint32_t w, uint32_t h, uint32_t log2s)
 // spatial variance of young pixels runs inside the first a-trous iteration). log2 stride of
 // stage i:
 SVGF_HD inline uint32_t stage_log2s(int /*canonical*/, int stage) { return stage < 1 ? 0u : (uint32_t)(stage - 1); }
-// CANONICAL a-trous iterations at strides s >= 8 run residue pairs: one workgroup filters the
+// CANONICAL a-trous iterations at strides s >= 2 run residue pairs: one workgroup filters the
 // lattices of residues (rx, ry) and (rx + 1, ry) (rx even), whose entries are adjacent pixels.
 // A halo entry then brings two pixels of one cache line instead of one, which halves the L2 ->
-// L1 requests that dominate these strides (each lattice row touches one line per entry). Such
-// a stage has s^2 / 2 * ceil(ceil(w/s)/16) * ceil(ceil(h/s)/16) tiles.
-constexpr uint32_t PAIR_LOG2S = 3;
+// L1 requests of the large strides (each lattice row touches one line per entry), and the
+// workgroup's 256 threads each filter two lattice points of one residue that share most of
+// their taps. Such a stage has s^2 / 2 * ceil(ceil(w/s)/16) * ceil(ceil(h/s)/16) tiles.
+constexpr uint32_t PAIR_LOG2S = 1;
 
 SVGF_HD inline bool paired_stage(int canonical, int stage, uint32_t log2s)
 {
