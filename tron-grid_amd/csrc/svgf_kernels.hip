@@ -224,6 +224,14 @@ __device__ __forceinline__ Tile decode_pair_tile(int w, uint32_t log2s, uint32_t
     return Tile{(int)log2s, (int)(2 * (res & (s / 2 - 1))), (int)(res >> (log2s - 1)), (int)(tx * LT), (int)(ty * LT)};
 }
 
+// Tile t of the first CANONICAL a-trous iteration (stride 1): 16 x 32 pixels, row-major.
+__device__ __forceinline__ Tile decode_tall_tile(int w, uint32_t t)
+{
+    const uint32_t ntx = ((uint32_t)w + LT - 1) / LT;
+    const uint32_t ty = t / ntx, tx = t - ty * ntx;
+    return Tile{0, 0, 0, (int)(tx * LT), (int)(ty * 2 * LT)};
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Renumber so
 // that XCD x runs one contiguous range of logical tiles: neighbouring tiles, which share
 // halo rows and (for s > 1) cache lines of other residues, then meet in the same L2.
@@ -591,32 +599,41 @@ struct Sums {
 // A-trous iteration k (stride s = 2^k) on one lattice tile: luminance, depth and normal
 // edge-stops, variance filtered by squared weights. Iteration 0 also writes the colour
 // history, the last one the output.
-// FIRST (iteration 0, stride 1) first replaces the temporal variance of young pixels (n < 4)
-// by their spatial variance. The taps and the variance prefilter read the variance of every
-// entry of the 20x20 tap region, and a young entry's 3x3 window reaches one pixel further, so
-// this iteration stages a 22x22 region; the young entries of the tap region are listed and
-// the block's threads compute their windows from LDS before the filter runs.
+// FIRST (iteration 0, stride 1) runs on a tall 16x32 pixel tile (TALL; a 16x16 tile for single
+// small frames) and first replaces the temporal variance of young pixels (n < 4) by their
+// spatial variance. The taps and the variance prefilter read the variance of every entry of
+// the 20x36 (20x20) tap region, and a young entry's 3x3 window reaches one pixel further, so
+// this iteration stages a 22x38 (22x22) region; the young entries of the tap region are listed
+// and the block's threads compute their windows from LDS before the filter runs.
 // NR = 2: the tile holds residues (rx, ry) and (rx + 1, ry) (strides >= 2): the region of each
 // is staged (the pair's two pixels of an entry are loaded by neighbouring lanes, one cache line);
-// each half of the block filters one residue, a thread two vertically adjacent lattice points.
-template <class S, bool FIRST, int NR>
+// each half of the block filters one residue.
+// TALL and NR = 2 (blocked): a thread filters two vertically adjacent lattice points, whose 5x5
+// taps share 4 of their 5 rows: 6 rows of LDS entries serve both. Otherwise (the 16x16 tiles
+// of single small frames) a thread filters one lattice point.
+template <class S, bool FIRST, int NR, bool TALL>
 __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
 {
     static_assert(!FIRST || NR == 1, "the first iteration runs at stride 1");
-    constexpr int H = FIRST ? LHALO + 1 : LHALO;  // staged halo
-    constexpr int R = LT + 2 * H;                 // staged region width
-    __shared__ float4 sA[NR * R * R];  // (r, g, b, luminance), residue-major
-    __shared__ float4 sB[NR * R * R];  // (normal, depth)
-    __shared__ float sV[NR * R * R];   // variance
-    __shared__ int sList[FIRST ? LREG * LREG : 1];
+    static_assert(!TALL || FIRST, "tall tiles are the first iteration's");
+    constexpr bool BLK = TALL || NR == 2;
+    constexpr int H = FIRST ? LHALO + 1 : LHALO;          // staged halo
+    constexpr int TH = TALL ? 2 * LT : LT;                // tile height (lattice points)
+    constexpr int RW = LT + 2 * H, RH = TH + 2 * H;       // staged region
+    constexpr int RR = RW * RH;
+    constexpr int QW = LT + 2 * LHALO, QH = TH + 2 * LHALO;  // tap region
+    __shared__ float4 sA[NR * RR];  // (r, g, b, luminance), residue-major
+    __shared__ float4 sB[NR * RR];  // (normal, depth)
+    __shared__ float sV[NR * RR];   // variance
+    __shared__ uint16_t sList[FIRST ? QW * QH : 1];
     __shared__ int sCount;
     const int s = 1 << k;
     const Buf src = bind((k & 1) ? f.wb : f.wa, f.h), gb = bind(f.g, f.h);
     const int max_x = f.w - 1, max_y = f.h - 1;
     const int tid = lane_tid();
-    // FIRST: history lengths of the tap region's entries tid and tid + 256, loaded together with
-    // the staged region (no extra round trip)
-    constexpr int NQ = FIRST ? (LREG * LREG + BLOCK - 1) / BLOCK : 0;
+    // FIRST: history lengths of the tap region's entries tid + 256 j, loaded together with the
+    // staged region (no extra round trip)
+    constexpr int NQ = FIRST ? (QW * QH + BLOCK - 1) / BLOCK : 0;
     uint32_t nq[NQ > 0 ? NQ : 1];
     if constexpr (FIRST) {
         if (tid == 0) sCount = 0;
@@ -624,20 +641,20 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
             const int e = tid + j * BLOCK;
-            const int qx = clampi(T.i0 - LHALO + e % LREG, 0, max_x), qy = clampi(T.j0 - LHALO + e / LREG, 0, max_y);
-            nq[j] = e < LREG * LREG ? ld_u8(nl, qy * nl.pitch + qx) : 4u;
+            const int qx = clampi(T.i0 - LHALO + e % QW, 0, max_x), qy = clampi(T.j0 - LHALO + e / QW, 0, max_y);
+            nq[j] = e < QW * QH ? ld_u8(nl, qy * nl.pitch + qx) : 4u;
         }
     }
     {
         // every load of the region is issued before the first LDS write (one memory round trip);
         // the last pass's surplus lanes load the final entry again and store nothing
-        constexpr int NE = NR * R * R, NIT = (NE + BLOCK - 1) / BLOCK;
+        constexpr int NE = NR * RR, NIT = (NE + BLOCK - 1) / BLOCK;
         float4 va[NIT], vb[NIT];
 #pragma unroll
         for (int j = 0; j < NIT; ++j) {
             const int e2 = min(tid + j * BLOCK, NE - 1);
             const int r = e2 % NR, e = e2 / NR;  // neighbouring lanes: the two residues of one entry
-            const int li = e % R, lj = e / R;
+            const int li = e % RW, lj = e / RW;
             const int qx = clampi(T.px(T.i0 - H + li) + r, 0, max_x), qy = clampi(T.py(T.j0 - H + lj), 0, max_y);
             va[j] = WavS::ld4(src, qx, qy);
             vb[j] = ldg4(gb, qx, qy);
@@ -648,9 +665,9 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             if (j + 1 < NIT || e2 < NE) {
                 const int r = e2 % NR, e = e2 / NR;
                 const float4 v = va[j];
-                sA[r * R * R + e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
-                sB[r * R * R + e] = vb[j];
-                sV[r * R * R + e] = v.w * v.w;
+                sA[r * RR + e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
+                sB[r * RR + e] = vb[j];
+                sV[r * RR + e] = v.w * v.w;
             }
         }
     }
@@ -667,7 +684,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             if ((tid & 63) == 0 && m) base = atomicAdd(&sCount, (int)__popcll(m));
             base = __builtin_amdgcn_readfirstlane(base);
             const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (yng) sList[base + rank] = tid + j * BLOCK;
+            if (yng) sList[base + rank] = (uint16_t)(tid + j * BLOCK);
         }
         lds_writes_done();
         __syncthreads();
@@ -675,16 +692,16 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         const Buf gr = bind(f.grad, f.h);
         for (int i = tid; i < count; i += BLOCK) {
             const int e = sList[i];
-            const int qx = clampi(T.i0 - LHALO + e % LREG, 0, max_x), qy = clampi(T.j0 - LHALO + e / LREG, 0, max_y);
+            const int qx = clampi(T.i0 - LHALO + e % QW, 0, max_x), qy = clampi(T.j0 - LHALO + e / QW, 0, max_y);
             // staged entry (li, lj) holds pixel (clamp(i0 - 3 + li), clamp(j0 - 3 + lj)), so
             // pixel (clamp(qx + dx), clamp(qy + dy)) sits at its offset from (i0 - 3, j0 - 3)
-            const int ec = (qy - (T.j0 - H)) * R + (qx - (T.i0 - H));
+            const int ec = (qy - (T.j0 - H)) * RW + (qx - (T.i0 - H));
             const float2 gxy = ld_f2(gr, qy * gr.pitch + qx * 8);
             const float var = young_variance(p, sA, sB, sB[ec], gxy.x, gxy.y, [&](int dx, int dy) {
-                return (clampi(qy + dy, 0, max_y) - (T.j0 - H)) * R + (clampi(qx + dx, 0, max_x) - (T.i0 - H));
+                return (clampi(qy + dy, 0, max_y) - (T.j0 - H)) * RW + (clampi(qx + dx, 0, max_x) - (T.i0 - H));
             });
             const float sd = sqrt_hw(var);  // the value W0 would hold
-            sV[(e / LREG + 1) * R + e % LREG + 1] = sd * sd;
+            sV[(e / QW + H - LHALO) * RW + e % QW + H - LHALO] = sd * sd;
         }
         lds_writes_done();
     }
@@ -696,9 +713,10 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
             for (int dx = -1; dx <= 1; ++dx)
-                gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c + dy * R + dx], gv);
+                gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c + dy * RW + dx], gv);
         return gv;
     };
+    // the per-pixel depth gradient the temporal stage stored
     const Buf gr = bind(f.grad, f.h);
     auto store = [&](int x, int y, float4 o) {
         if (k == 0) {
@@ -709,13 +727,12 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         if (k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
         else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
     };
-    if constexpr (NR == 1) {
+    if constexpr (!BLK) {
         const int ti = tid & (LT - 1), tj = tid >> 4;
         const int x = T.px(T.i0 + ti), y = T.py(T.j0 + tj);
         if (x <= max_x && y <= max_y) {
-            const int c0 = (tj + H) * R + ti + H;
+            const int c0 = (tj + H) * RW + ti + H;
             const float4 C = sA[c0];
-            // the per-pixel depth gradient the temporal stage stored
             const float2 g = ld_f2(gr, y * gr.pitch + x * 8);
             const Centre c = centre_of(p, C, sB[c0], prefilter(c0), g.x, g.y, s);
             Sums a;
@@ -723,7 +740,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             for (int dy = -2; dy <= 2; ++dy) {
 #pragma unroll
                 for (int dx = -2; dx <= 2; ++dx) {
-                    const int q = c0 + dy * R + dx;
+                    const int q = c0 + dy * RW + dx;
                     const float4 A = sA[q];
                     a.add(tap_weight(p, c, dx, dy, A, sB[q]), A, sV[q]);
                 }
@@ -732,13 +749,11 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             store(x, y, a.result(C, sV[c0]));
         }
     } else {
-        // residue r = tid / 128; the thread filters the vertically adjacent lattice points
-        // (ti, 2u) and (ti, 2u + 1), whose 5x5 taps share 4 of their 5 rows: 6 rows of LDS
-        // entries serve both.
-        const int r = tid >> 7, l = tid & 127, ti = l & (LT - 1), u = l >> 4;
+        // residue r (NR = 2: tid / 128); lattice points (ti, 2u) and (ti, 2u + 1) of the tile
+        const int r = NR == 2 ? tid >> 7 : 0, l = NR == 2 ? tid & 127 : tid, ti = l & (LT - 1), u = l >> 4;
         const int x = T.px(T.i0 + ti) + r, y0 = T.py(T.j0 + 2 * u), y1 = y0 + s;
         if (x <= max_x && y0 <= max_y) {
-            const int c0 = r * R * R + (2 * u + H) * R + ti + H, c1 = c0 + R;
+            const int c0 = r * RR + (2 * u + H) * RW + ti + H, c1 = c0 + RW;
             const float4 C0 = sA[c0], C1 = sA[c1];
             const float2 g0 = ld_f2(gr, y0 * gr.pitch + x * 8);
             const float2 g1 = ld_f2(gr, min(y1, max_y) * gr.pitch + x * 8);  // unused past the frame
@@ -749,7 +764,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             for (int j = -2; j <= 3; ++j) {
 #pragma unroll
                 for (int dx = -2; dx <= 2; ++dx) {
-                    const int q = c0 + j * R + dx;
+                    const int q = c0 + j * RW + dx;
                     const float4 A = sA[q], B = sB[q];
                     const float V = sV[q];
                     if (j <= 2) a0.add(tap_weight(p, e0, dx, j, A, B), A, V);
@@ -769,7 +784,8 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
 // Kernels
 // ========================================================================================
 enum StageId : int {
-    ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_ATROUS0 = 3, ST_CAN_ATROUS = 4, ST_CAN_ATROUS_PAIR = 5
+    ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_ATROUS0 = 3, ST_CAN_ATROUS = 4, ST_CAN_ATROUS_PAIR = 5,
+    ST_CAN_ATROUS0_SQ = 6  // the first a-trous iteration on 16x16 tiles (single small frames)
 };
 
 template <class S, int STAGE, bool COMP>
@@ -778,9 +794,10 @@ __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, 
     if constexpr (STAGE == ST_REF_INIT) ref_init_tile<S>(f, p, T);
     else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_tile<S>(f, p, T, arg);
     else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, COMP>(f, p, T);
-    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, true, 1>(f, p, T, 0);
-    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, false, 1>(f, p, T, arg);
-    else can_atrous_tile<S, false, 2>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, true, 1, true>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS0_SQ) can_atrous_tile<S, true, 1, false>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, false, 1, false>(f, p, T, arg);
+    else can_atrous_tile<S, false, 2, false>(f, p, T, arg);
 }
 
 // Frame descriptor of a batch entry into scalar registers (the index is uniform across the block).
@@ -799,6 +816,7 @@ template <int STAGE>
 __device__ __forceinline__ Tile stage_tile(int w, uint32_t log2s, uint32_t t)
 {
     if constexpr (STAGE == ST_CAN_ATROUS_PAIR) return decode_pair_tile(w, log2s, t);
+    else if constexpr (STAGE == ST_CAN_ATROUS0) return decode_tall_tile(w, t);
     else return decode_tile(w, log2s, t);
 }
 
@@ -929,7 +947,7 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
             if constexpr (CANON) {
                 if (k.stage == 0) persist_tile<S, ST_CAN_TEMPORAL, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
                 else if (k.stage == 1) persist_tile<S, ST_CAN_ATROUS0, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
-                else if (k.stage - 1 < PAIR_LOG2S)
+                else if (PAIR_LOG2S > 1 && k.stage - 1 < PAIR_LOG2S)
                     persist_tile<S, ST_CAN_ATROUS, COMP>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
                 else persist_tile<S, ST_CAN_ATROUS_PAIR, COMP>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
             } else {
@@ -965,8 +983,10 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
 template <class S, int STAGE, bool COMP>
 void launch_frame(const FrameDesc& f, const KParams& p, int arg, uint32_t log2s, hipStream_t stream)
 {
-    uint32_t n = STAGE == ST_CAN_ATROUS_PAIR ? pair_tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s)
-                                             : tiles_at((uint32_t)f.w, (uint32_t)f.h, log2s);
+    const uint32_t w = (uint32_t)f.w, h = (uint32_t)f.h;
+    const uint32_t n = STAGE == ST_CAN_ATROUS_PAIR ? pair_tiles_at(w, h, log2s)
+                       : STAGE == ST_CAN_ATROUS0   ? tall_tiles_at(w, h)
+                                                   : tiles_at(w, h, log2s);
     hipLaunchKernelGGL((k_frame<S, STAGE, COMP>), dim3(n), dim3(BLOCK), 0, stream, f, p, arg, log2s);
 }
 
@@ -1000,11 +1020,12 @@ void canonical_frame(const FrameDesc& f, const KParams& p, hipStream_t s, hipEve
     mark(ev, 0, s);
     launch_frame<S, ST_CAN_TEMPORAL, COMP>(f, p, 0, 0, s);
     mark(ev, 1, s);
-    launch_frame<S, ST_CAN_ATROUS0, COMP>(f, p, 0, 0, s);
+    // A single small frame has few tiles: halving them (tall tiles, residue pairs) would leave
+    // the device emptier, so both are used from 2048 16x16 single-residue tiles on.
+    if (tiles_at((uint32_t)f.w, (uint32_t)f.h, 0) < 2048u) launch_frame<S, ST_CAN_ATROUS0_SQ, COMP>(f, p, 0, 0, s);
+    else launch_frame<S, ST_CAN_ATROUS0, COMP>(f, p, 0, 0, s);
     mark(ev, 2, s);
     for (int k = 1; k < p.iterations; ++k) {
-        // A single small frame has few tiles at large strides: halving them (residue pairs)
-        // would leave the device emptier, so pairs are used from 2048 single-residue tiles on.
         if (k < (int)PAIR_LOG2S || tiles_at((uint32_t)f.w, (uint32_t)f.h, (uint32_t)k) < 2048u)
             launch_frame<S, ST_CAN_ATROUS, COMP>(f, p, k, (uint32_t)k, s);
         else launch_frame<S, ST_CAN_ATROUS_PAIR, COMP>(f, p, k, (uint32_t)k, s);

@@ -55,10 +55,19 @@ SVGF_HD inline uint32_t pair_tiles_at(uint32_t w, uint32_t h, uint32_t log2s)
     return tiles_at(w, h, log2s) / 2;
 }
 
+// The first CANONICAL a-trous iteration (stride 1, with the spatial variance of young pixels)
+// runs on tall 16 x 32 pixel tiles, whose 256 threads each filter two vertically adjacent
+// pixels: ceil(w/16) * ceil(h/32) tiles.
+SVGF_HD inline uint32_t tall_tiles_at(uint32_t w, uint32_t h)
+{
+    return ((w + LT - 1) / LT) * ((h + 2 * LT - 1) / (2 * LT));
+}
+
 // Tiles (workgroups) of stage `stage` for a w x h frame.
 SVGF_HD inline uint32_t stage_tiles(int canonical, int stage, uint32_t w, uint32_t h)
 {
     const uint32_t log2s = stage_log2s(canonical, stage);
+    if (canonical && stage == 1) return tall_tiles_at(w, h);
     return paired_stage(canonical, stage, log2s) ? pair_tiles_at(w, h, log2s) : tiles_at(w, h, log2s);
 }
 
