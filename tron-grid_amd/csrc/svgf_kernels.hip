@@ -343,22 +343,17 @@ __device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(
 // variance)) into wavelet_a, the history length (uint8) and the moments; the batched form
 // also keeps G for the next frame. Standard deviations rather than variances are stored so
 // that heavy-tailed radiance stays inside the FP16 range.
+// Pixel (x, y) of the frame, with its G and the depths of its four clamped neighbours.
 template <class S, bool COMP>
-__device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Tile& T)
+__device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KParams& p, int x, int y, float4 G,
+                                                   float zl, float zr, float zu, float zd)
 {
-    const int tid = lane_tid();
-    const int x = T.i0 + (tid & (LT - 1)), y = T.j0 + (tid >> 4);
-    if (x >= f.w || y >= f.h) return;
-    const Buf rad = bind(f.rad, f.h), gb = bind(f.g, f.h);
-    float4 R = S::ld4(rad, x, y);
+    float4 R = S::ld4(bind(f.rad, f.h), x, y);
     float l = luma_fma(R.x, R.y, R.z);
-    float4 G = ldg4(gb, x, y);
     if (f.gsave.ptr) stg4(bind(f.gsave, f.h), x, y, G);
     {
         // depth gradient (central differences of clamped neighbours, halved), kept for the
-        // variance and a-trous stages of this frame
-        float zl = ldgz(gb, clampi(x - 1, 0, f.w - 1), y), zr = ldgz(gb, clampi(x + 1, 0, f.w - 1), y);
-        float zu = ldgz(gb, x, clampi(y - 1, 0, f.h - 1)), zd = ldgz(gb, x, clampi(y + 1, 0, f.h - 1));
+        // a-trous stages of this frame
         const Buf gr = bind(f.grad, f.h);
         st_f2(gr, y * gr.pitch + x * 8, make_float2(0.5f * (zr - zl), 0.5f * (zd - zu)));
     }
@@ -464,6 +459,44 @@ __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KPar
     WavS::st4(bind(f.wa, f.h), x, y, make_float4(c0, c1, c2, sqrt_hw(var)));
     const Buf nl = bind(f.nlen, f.h);
     st_u8(nl, y * nl.pitch + x, (uint32_t)n_new);
+}
+
+template <class S, bool COMP>
+__device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Tile& T)
+{
+    const int tid = lane_tid();
+    const int ti = tid & (LT - 1), tj = tid >> 4;
+    const int x = T.i0 + ti, y = T.j0 + tj;
+    const int max_x = f.w - 1, max_y = f.h - 1;
+    const Buf gb = bind(f.g, f.h);
+    // Depths of the tile and a one-pixel border, clamped into the frame, shared through LDS for
+    // the depth gradient: one G load per pixel instead of a G load and four depth loads. Lanes
+    // past the frame edge load the clamped pixel, which is what their neighbours' clamped
+    // lookups read.
+    constexpr int ZW = LT + 2;
+    __shared__ float sZ[ZW * ZW];
+    const float4 G = ldg4(gb, min(x, max_x), min(y, max_y));
+    sZ[(tj + 1) * ZW + ti + 1] = G.w;
+    if (tid < 4 * LT + 4) {
+        // border: the rows above and below (ZW entries each), then the columns left and right
+        int bx, by;
+        if (tid < 2 * ZW) {
+            bx = tid % ZW - 1;
+            by = tid < ZW ? -1 : LT;
+        } else {
+            const int c = tid - 2 * ZW;
+            bx = (c & 1) ? LT : -1;
+            by = c >> 1;
+        }
+        sZ[(by + 1) * ZW + bx + 1] = ldgz(gb, clampi(T.i0 + bx, 0, max_x), clampi(T.j0 + by, 0, max_y));
+    }
+    lds_writes_done();
+    __syncthreads();
+    if (x <= max_x && y <= max_y) {
+        const int c = (tj + 1) * ZW + ti + 1;
+        can_temporal_pixel<S, COMP>(f, p, x, y, G, sZ[c - 1], sZ[c + 1], sZ[c - ZW], sZ[c + ZW]);
+    }
+    __syncthreads();  // the next tile of a grid-stride loop reuses sZ
 }
 
 // Spatial variance of a young pixel (history length n < 4): the edge-aware variance of the
