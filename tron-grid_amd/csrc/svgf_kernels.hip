@@ -78,6 +78,19 @@ struct F32S {
     __device__ static __forceinline__ void st4(const Buf& b, int x, int y, float4 v) { st_f4(b, y * b.pitch + x * B4, v); }
     __device__ static __forceinline__ float2 ld2(const Buf& b, int x, int y) { return ld_f2(b, y * b.pitch + x * B2); }
     __device__ static __forceinline__ void st2(const Buf& b, int x, int y, float2 v) { st_f2(b, y * b.pitch + x * B2, v); }
+    // pixels x and x + 1 of row y, v1 = v0 when `first` (a read past the plane's end returns
+    // zeros)
+    __device__ static __forceinline__ void ld4x2(const Buf& b, int x, int y, bool first, float4& v0, float4& v1)
+    {
+        v0 = ld_f4(b, y * b.pitch + x * B4);
+        v1 = ld_f4(b, y * b.pitch + x * B4 + (first ? 0 : B4));
+    }
+    __device__ static __forceinline__ void ld2x2(const Buf& b, int x, int y, bool first, float2& v0, float2& v1)
+    {
+        const float4 v = ld_f4(b, y * b.pitch + x * B2);
+        v0 = make_float2(v.x, v.y);
+        v1 = first ? v0 : make_float2(v.z, v.w);
+    }
     // alpha channel alone
     __device__ static __forceinline__ float lda(const Buf& b, int x, int y) { return ld_f1(b, y * b.pitch + x * B4 + 12); }
     __device__ static __forceinline__ void sta(const Buf& b, int x, int y, float v) { st_f1(b, y * b.pitch + x * B4 + 12, v); }
@@ -104,6 +117,23 @@ struct F16S {
     {
         half2_t h = {(_Float16)v.x, (_Float16)v.y};
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, h), b.r, y * b.pitch + x * B2, 0, 0);
+    }
+    // pixels x and x + 1 of row y in one load, v1 = v0 when `first` (selected before the
+    // conversion; a read past the plane's end returns zeros)
+    __device__ static __forceinline__ void ld4x2(const Buf& b, int x, int y, bool first, float4& v0, float4& v1)
+    {
+        const u32x4_t r = __builtin_amdgcn_raw_buffer_load_b128(b.r, y * b.pitch + x * B4, 0, 0);
+        const half4_t h0 = __builtin_bit_cast(half4_t, u32x2_t{r.x, r.y});
+        const half4_t h1 = __builtin_bit_cast(half4_t, first ? u32x2_t{r.x, r.y} : u32x2_t{r.z, r.w});
+        v0 = make_float4((float)h0.x, (float)h0.y, (float)h0.z, (float)h0.w);
+        v1 = make_float4((float)h1.x, (float)h1.y, (float)h1.z, (float)h1.w);
+    }
+    __device__ static __forceinline__ void ld2x2(const Buf& b, int x, int y, bool first, float2& v0, float2& v1)
+    {
+        const u32x2_t r = __builtin_amdgcn_raw_buffer_load_b64(b.r, y * b.pitch + x * B2, 0, 0);
+        const half2_t h0 = __builtin_bit_cast(half2_t, r.x), h1 = __builtin_bit_cast(half2_t, first ? r.x : r.y);
+        v0 = make_float2((float)h0.x, (float)h0.y);
+        v1 = make_float2((float)h1.x, (float)h1.y);
     }
     __device__ static __forceinline__ float lda(const Buf& b, int x, int y)
     {
@@ -371,15 +401,20 @@ __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KPa
         int x0 = x + (int)flx, y0 = y + (int)fly;
         // Issue the G / history / moment loads of all four taps at once (addresses clamped into
         // the frame; taps outside it are discarded below) so the reprojection costs one
-        // dependent memory round trip instead of three.
+        // dependent memory round trip instead of three. History and moments come as the pixel
+        // pair (px, px + 1) of each row, px = clamp(x0): when x0 = -1 the right tap (x = 0) is
+        // the pair's first pixel; every other in-frame tap is the pixel it names.
         float4 Gq[4], H[4];
         float2 M[4];
+        const int px = clampi(x0, 0, f.w - 1);
+        const bool right_first = x0 < 0;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            int cx = clampi(x0 + (t & 1), 0, f.w - 1), cy = clampi(y0 + (t >> 1), 0, f.h - 1);
-            Gq[t] = ldg4(gp, cx, cy);
-            H[t] = S::ld4(hp, cx, cy);
-            M[t] = S::ld2(mp, cx, cy);
+        for (int jj = 0; jj < 2; ++jj) {
+            const int cy = clampi(y0 + jj, 0, f.h - 1);
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) Gq[jj * 2 + ii] = ldg4(gp, clampi(x0 + ii, 0, f.w - 1), cy);
+            S::ld4x2(hp, px, cy, right_first, H[jj * 2], H[jj * 2 + 1]);
+            S::ld2x2(mp, px, cy, right_first, M[jj * 2], M[jj * 2 + 1]);
         }
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
