@@ -757,17 +757,19 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         lds_writes_done();
         __syncthreads();
         const int count = sCount;
-        const Buf gr = bind(f.grad, f.h);
         for (int i = tid; i < count; i += BLOCK) {
             const int e = sList[i];
             const int qx = clampi(T.i0 - LHALO + e % QW, 0, max_x), qy = clampi(T.j0 - LHALO + e / QW, 0, max_y);
             // staged entry (li, lj) holds pixel (clamp(i0 - 3 + li), clamp(j0 - 3 + lj)), so
             // pixel (clamp(qx + dx), clamp(qy + dy)) sits at its offset from (i0 - 3, j0 - 3)
             const int ec = (qy - (T.j0 - H)) * RW + (qx - (T.i0 - H));
-            const float2 gxy = ld_f2(gr, qy * gr.pitch + qx * 8);
-            const float var = young_variance(p, sA, sB, sB[ec], gxy.x, gxy.y, [&](int dx, int dy) {
+            auto tap = [&](int dx, int dy) {
                 return (clampi(qy + dy, 0, max_y) - (T.j0 - H)) * RW + (clampi(qx + dx, 0, max_x) - (T.i0 - H));
-            });
+            };
+            // the depth gradient of the entry from the staged depths of its clamped neighbours
+            // (the temporal stage's central differences, the same floats)
+            const float gx = 0.5f * (sB[tap(1, 0)].w - sB[tap(-1, 0)].w), gy = 0.5f * (sB[tap(0, 1)].w - sB[tap(0, -1)].w);
+            const float var = young_variance(p, sA, sB, sB[ec], gx, gy, tap);
             const float sd = sqrt_hw(var);  // the value W0 would hold
             sV[(e / QW + H - LHALO) * RW + e % QW + H - LHALO] = sd * sd;
         }
@@ -801,7 +803,8 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         if (x <= max_x && y <= max_y) {
             const int c0 = (tj + H) * RW + ti + H;
             const float4 C = sA[c0];
-            const float2 g = ld_f2(gr, y * gr.pitch + x * 8);
+            const float2 g = FIRST ? make_float2(0.5f * (sB[c0 + 1].w - sB[c0 - 1].w), 0.5f * (sB[c0 + RW].w - sB[c0 - RW].w))
+                                   : ld_f2(gr, y * gr.pitch + x * 8);
             const Centre c = centre_of(p, C, sB[c0], prefilter(c0), g.x, g.y, s);
             Sums a;
 #pragma unroll
@@ -823,8 +826,16 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         if (x <= max_x && y0 <= max_y) {
             const int c0 = r * RR + (2 * u + H) * RW + ti + H, c1 = c0 + RW;
             const float4 C0 = sA[c0], C1 = sA[c1];
-            const float2 g0 = ld_f2(gr, y0 * gr.pitch + x * 8);
-            const float2 g1 = ld_f2(gr, min(y1, max_y) * gr.pitch + x * 8);  // unused past the frame
+            float2 g0, g1;
+            if constexpr (FIRST) {
+                // stride 1: the clamped neighbours are staged entries (the temporal stage's
+                // central differences, the same floats)
+                g0 = make_float2(0.5f * (sB[c0 + 1].w - sB[c0 - 1].w), 0.5f * (sB[c1].w - sB[c0 - RW].w));
+                g1 = make_float2(0.5f * (sB[c1 + 1].w - sB[c1 - 1].w), 0.5f * (sB[c1 + RW].w - sB[c0].w));
+            } else {
+                g0 = ld_f2(gr, y0 * gr.pitch + x * 8);
+                g1 = ld_f2(gr, min(y1, max_y) * gr.pitch + x * 8);  // unused past the frame
+            }
             const Centre e0 = centre_of(p, C0, sB[c0], prefilter(c0), g0.x, g0.y, s);
             const Centre e1 = centre_of(p, C1, sB[c1], prefilter(c1), g1.x, g1.y, s);
             Sums a0, a1;
