@@ -716,26 +716,29 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     {
         // every load of the region is issued before the first LDS write (one memory round trip);
         // the last pass's surplus lanes load the final entry again and store nothing
-        constexpr int NE = NR * RR, NIT = (NE + BLOCK - 1) / BLOCK;
+        constexpr uint32_t NE = NR * RR, NIT = (NE + BLOCK - 1) / BLOCK;
         float4 va[NIT], vb[NIT];
+        uint32_t slot[NIT];  // LDS index of the entry
+        const int bx = T.i0 - H, by = T.j0 - H;
 #pragma unroll
-        for (int j = 0; j < NIT; ++j) {
-            const int e2 = min(tid + j * BLOCK, NE - 1);
-            const int r = e2 % NR, e = e2 / NR;  // neighbouring lanes: the two residues of one entry
-            const int li = e % RW, lj = e / RW;
-            const int qx = clampi(T.px(T.i0 - H + li) + r, 0, max_x), qy = clampi(T.py(T.j0 - H + lj), 0, max_y);
+        for (uint32_t j = 0; j < NIT; ++j) {
+            // unsigned index arithmetic: divisions by the constant widths are a multiply-high
+            const uint32_t e2 = j + 1 < NIT ? (uint32_t)tid + j * BLOCK : min((uint32_t)tid + j * BLOCK, NE - 1);
+            const uint32_t r = e2 % NR, e = e2 / NR;  // neighbouring lanes: the two residues of one entry
+            const uint32_t li = e % RW, lj = e / RW;
+            slot[j] = r * RR + e;
+            const int qx = clampi(T.rx + ((bx + (int)li) << T.log2s) + (int)r, 0, max_x);
+            const int qy = clampi(T.ry + ((by + (int)lj) << T.log2s), 0, max_y);
             va[j] = WavS::ld4(src, qx, qy);
             vb[j] = ldg4(gb, qx, qy);
         }
 #pragma unroll
-        for (int j = 0; j < NIT; ++j) {
-            const int e2 = tid + j * BLOCK;
-            if (j + 1 < NIT || e2 < NE) {
-                const int r = e2 % NR, e = e2 / NR;
+        for (uint32_t j = 0; j < NIT; ++j) {
+            if (j + 1 < NIT || (uint32_t)tid + j * BLOCK < NE) {
                 const float4 v = va[j];
-                sA[r * RR + e] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
-                sB[r * RR + e] = vb[j];
-                sV[r * RR + e] = v.w * v.w;
+                sA[slot[j]] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
+                sB[slot[j]] = vb[j];
+                sV[slot[j]] = v.w * v.w;
             }
         }
     }
