@@ -43,14 +43,12 @@ def progress(msg: str) -> None:
 #   a-trous  : wavelet (FP32) + G + depth gradient read; wavelet written (the last iteration
 #              writes the output instead); iteration 0 also reads the length and writes the
 #              colour history, and computes the spatial variance of young pixels (n < 4) from
-#              the region it already holds in LDS (no extra plane). Iteration 0 of the batched
-#              form (16x32 tiles) takes the gradient from the staged depths instead of the
-#              gradient plane; the single-frame form does too from 2048 16x16 tiles on, and the
-#              plain figures count the plane read (the smaller frames of the mix).
+#              the region it already holds in LDS (no extra plane); at stride 1 it also takes
+#              the depth gradient from the staged depths instead of the gradient plane
 def stage_bytes(storage_bytes4, storage_bytes2, iterations, batched):
     s4, s2, w4 = storage_bytes4, storage_bytes2, 16
     temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (w4 + s2 + 1 + 8) + (16 if batched else 0)
-    atrous = [(w4 + 16) + (0 if (k == 0 and batched) else 8) + (s4 if k == iterations - 1 else w4)
+    atrous = [(w4 + 16) + (0 if k == 0 else 8) + (s4 if k == iterations - 1 else w4)
               + ((1 + s4) if k == 0 else 0) for k in range(iterations)]
     return [temporal] + atrous
 
