@@ -15,12 +15,13 @@
 
 namespace svgf {
 
-// Every stage runs on 16x16 tiles of a stride-s lattice: for a-trous iteration k (s = 2^k) the
-// pixels of one residue class (x mod s, y mod s) form a lattice whose 5x5 taps are the lattice
-// neighbours, so a tile plus a 2-point halo (20x20 entries) holds every tap in LDS whatever the
-// stride. Pointwise stages use s = 1 (plain pixel tiles). A frame at stride s has
+// Stages run on tiles of a stride-s lattice: for a-trous iteration k (s = 2^k) the pixels of one
+// residue class (x mod s, y mod s) form a lattice whose 5x5 taps are the lattice neighbours, so a
+// tile plus a 2-point halo holds every tap in LDS whatever the stride. Pointwise stages use s = 1
+// (plain pixel tiles). With 16x16 tiles a frame at stride s has
 // s^2 * ceil(ceil(w/s)/16) * ceil(ceil(h/s)/16) tiles, ordered tile-major with the residue
-// fastest (the s^2 blocks of one 16s x 16s pixel region are adjacent).
+// fastest (the s^2 blocks of one 16s x 16s pixel region are adjacent); the CANONICAL a-trous
+// iterations use the residue-pair and tall variants below.
 constexpr int LT = 16;
 constexpr int LHALO = 2;
 constexpr int LREG = LT + 2 * LHALO;  // 20
