@@ -791,12 +791,12 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     };
     // the per-pixel depth gradient the temporal stage stored
     const Buf gr = bind(f.grad, f.h);
-    auto store = [&](int x, int y, float4 o) {
-        if (k == 0) {
-            const Buf nl = bind(f.nlen, f.h);
-            const float n = (float)ld_u8(nl, y * nl.pitch + x);
-            S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, n));
-        }
+    // FIRST also writes the colour history: the output with the pixel's history length, loaded
+    // before the taps so that its latency hides behind them
+    const Buf nl = bind(f.nlen, f.h);
+    auto length = [&](int x, int y) { return FIRST ? ld_u8(nl, y * nl.pitch + x) : 0u; };
+    auto store = [&](int x, int y, float4 o, uint32_t n) {
+        if constexpr (FIRST) S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, (float)n));
         if (k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
         else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
     };
@@ -808,6 +808,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             const float4 C = sA[c0];
             const float2 g = FIRST ? make_float2(0.5f * (sB[c0 + 1].w - sB[c0 - 1].w), 0.5f * (sB[c0 + RW].w - sB[c0 - RW].w))
                                    : ld_f2(gr, y * gr.pitch + x * 8);
+            const uint32_t n = length(x, y);
             const Centre c = centre_of(p, C, sB[c0], prefilter(c0), g.x, g.y, s);
             Sums a;
 #pragma unroll
@@ -820,7 +821,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
                 }
                 a.pin();
             }
-            store(x, y, a.result(C, sV[c0]));
+            store(x, y, a.result(C, sV[c0]), n);
         }
     } else {
         // residue r (NR = 2: tid / 128); lattice points (ti, 2u) and (ti, 2u + 1) of the tile
@@ -839,6 +840,7 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
                 g0 = ld_f2(gr, y0 * gr.pitch + x * 8);
                 g1 = ld_f2(gr, min(y1, max_y) * gr.pitch + x * 8);  // unused past the frame
             }
+            const uint32_t n0 = length(x, y0), n1 = length(x, min(y1, max_y));
             const Centre e0 = centre_of(p, C0, sB[c0], prefilter(c0), g0.x, g0.y, s);
             const Centre e1 = centre_of(p, C1, sB[c1], prefilter(c1), g1.x, g1.y, s);
             Sums a0, a1;
@@ -855,8 +857,8 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
                 a0.pin();
                 a1.pin();
             }
-            store(x, y0, a0.result(C0, sV[c0]));
-            if (y1 <= max_y) store(x, y1, a1.result(C1, sV[c1]));
+            store(x, y0, a0.result(C0, sV[c0]), n0);
+            if (y1 <= max_y) store(x, y1, a1.result(C1, sV[c1]), n1);
         }
     }
     __syncthreads();  // the next tile of a grid-stride loop reuses the LDS arrays
