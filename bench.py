@@ -876,6 +876,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": round(dom_bytes_px * batch_px),
                      "algorithmic_bytes_per_px": dom_bytes_px, "mean_launch_ms": round(dom_ms, 5),
+                     "launch_timing": "hipEvents around each stage launch on the launch stream, in a pass over the "
+                                      "same batches after the timed one (per-stage events need launches outside "
+                                      "the graph); mean of the kernel's launches per batch",
                      "note": "arithmetic-bound: ~16 VALU instructions (2 transcendental) per tap, 25 taps per "
                              "pixel, with the VALU and LDS the busiest units; HBM fraction is not the binding limit"},
         "cpu_baseline": cpu,
