@@ -272,6 +272,50 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t n)
 }
 
 // ========================================================================================
+// LDS of the stage tiles
+// ========================================================================================
+// Bytes a stage tile stages in LDS: the a-trous tile (colour + luminance, G, variance per entry
+// of NR staged regions, the young list and its count), the REFERENCE a-trous halo, the
+// temporal stage's depth tile.
+__host__ __device__ constexpr uint32_t atrous_lds_bytes(bool first, int nr, bool tall)
+{
+    const uint32_t h = first ? LHALO + 1 : LHALO, th = tall ? 2 * LT : LT;
+    const uint32_t entries = (uint32_t)nr * (LT + 2 * h) * (th + 2 * h);
+    const uint32_t list = first ? (LT + 2 * LHALO) * (th + 2 * LHALO) * 2u : 0u;
+    return ((entries * 36u + list + 15u) & ~15u) + 16u;
+}
+constexpr uint32_t REF_ATROUS_LDS = LREG * LREG * 16;
+constexpr uint32_t TEMPORAL_LDS = (LT + 2) * (LT + 2) * 4;
+constexpr uint32_t cmax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+// A kernel that runs one stage gives its tile function arrays of its own (OwnLds). The
+// persistent kernel runs every stage in turn: its tile functions share one arena sized for the
+// largest (ArenaLds), so its LDS is one stage's rather than the sum of all of them.
+struct OwnLds {
+    template <uint32_t BYTES>
+    __device__ static __forceinline__ char* get()
+    {
+        __shared__ __align__(16) char m[BYTES];
+        return m;
+    }
+};
+constexpr uint32_t PERSIST_LDS_CAN =
+    cmax(cmax(atrous_lds_bytes(true, 1, true), atrous_lds_bytes(false, 2, false)),
+         cmax(atrous_lds_bytes(false, 1, false), TEMPORAL_LDS));
+__shared__ __align__(16) char g_persist_lds_can[PERSIST_LDS_CAN];
+__shared__ __align__(16) char g_persist_lds_ref[REF_ATROUS_LDS];
+template <bool CANON>
+struct ArenaLds {
+    template <uint32_t BYTES>
+    __device__ static __forceinline__ char* get()
+    {
+        static_assert(BYTES <= (CANON ? PERSIST_LDS_CAN : REF_ATROUS_LDS), "persistent LDS arena too small");
+        if constexpr (CANON) return g_persist_lds_can;
+        else return g_persist_lds_ref;
+    }
+};
+
+// ========================================================================================
 // REFERENCE stages (svgf_denoise.slang)
 // ========================================================================================
 
@@ -301,10 +345,10 @@ __device__ __forceinline__ void ref_init_tile(const FrameDesc& f, const KParams&
 
 // Passes 1..N (svgf_denoise.slang:209-274): 5x5 B3-spline taps at `stride`, clamp-to-edge,
 // weight K[dx] K[dy] exp(-|l_q - l_c| / max(phi * sigma_c, eps)), dy outer / dx inner.
-template <class S>
+template <class S, class L>
 __device__ __forceinline__ void ref_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int pass)
 {
-    __shared__ float4 sA[LREG * LREG];  // (r, g, b, luminance) of the tile + halo
+    float4* const sA = reinterpret_cast<float4*>(L::template get<REF_ATROUS_LDS>());  // (r, g, b, luminance) of the tile + halo
     const bool source_is_a = (pass & 1) == 1;
     const Buf src = bind(source_is_a ? f.wa : f.wb, f.h);
     const int max_x = f.w - 1, max_y = f.h - 1;
@@ -496,7 +540,7 @@ __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KPa
     st_u8(nl, y * nl.pitch + x, (uint32_t)n_new);
 }
 
-template <class S, bool COMP>
+template <class S, bool COMP, class L>
 __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
     const int tid = lane_tid();
@@ -509,7 +553,7 @@ __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KPar
     // past the frame edge load the clamped pixel, which is what their neighbours' clamped
     // lookups read.
     constexpr int ZW = LT + 2;
-    __shared__ float sZ[ZW * ZW];
+    float* const sZ = reinterpret_cast<float*>(L::template get<TEMPORAL_LDS>());
     const float4 G = ldg4(gb, min(x, max_x), min(y, max_y));
     sZ[(tj + 1) * ZW + ti + 1] = G.w;
     if (tid < 4 * LT + 4) {
@@ -679,7 +723,7 @@ struct Sums {
 // TALL and NR = 2 (blocked): a thread filters two vertically adjacent lattice points, whose 5x5
 // taps share 4 of their 5 rows: 6 rows of LDS entries serve both. Otherwise (the 16x16 tiles
 // of single small frames) a thread filters one lattice point.
-template <class S, bool FIRST, int NR, bool TALL>
+template <class S, bool FIRST, int NR, bool TALL, class L>
 __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
 {
     static_assert(!FIRST || NR == 1, "the first iteration runs at stride 1");
@@ -690,11 +734,14 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     constexpr int RW = LT + 2 * H, RH = TH + 2 * H;       // staged region
     constexpr int RR = RW * RH;
     constexpr int QW = LT + 2 * LHALO, QH = TH + 2 * LHALO;  // tap region
-    __shared__ float4 sA[NR * RR];  // (r, g, b, luminance), residue-major
-    __shared__ float4 sB[NR * RR];  // (normal, depth)
-    __shared__ float sV[NR * RR];   // variance
-    __shared__ uint16_t sList[FIRST ? QW * QH : 1];
-    __shared__ int sCount;
+    constexpr uint32_t NA = NR * RR, LIST = FIRST ? QW * QH : 0;
+    static_assert(((NA * 36 + LIST * 2 + 15) & ~15u) + 16 == atrous_lds_bytes(FIRST, NR, TALL), "LDS layout");
+    char* const lds = L::template get<atrous_lds_bytes(FIRST, NR, TALL)>();
+    float4* const sA = reinterpret_cast<float4*>(lds);                // (r, g, b, luminance), residue-major
+    float4* const sB = reinterpret_cast<float4*>(lds + NA * 16);      // (normal, depth)
+    float* const sV = reinterpret_cast<float*>(lds + NA * 32);        // variance
+    uint16_t* const sList = reinterpret_cast<uint16_t*>(lds + NA * 36);  // young entries
+    int& sCount = *reinterpret_cast<int*>(lds + ((NA * 36 + LIST * 2 + 15) & ~15u));
     const int s = 1 << k;
     const Buf src = bind((k & 1) ? f.wb : f.wa, f.h), gb = bind(f.g, f.h);
     const int max_x = f.w - 1, max_y = f.h - 1;
@@ -872,16 +919,16 @@ enum StageId : int {
     ST_CAN_ATROUS0_SQ = 6  // the first a-trous iteration on 16x16 tiles (single small frames)
 };
 
-template <class S, int STAGE, bool COMP>
+template <class S, int STAGE, bool COMP, class L = OwnLds>
 __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, const Tile& T, int arg)
 {
     if constexpr (STAGE == ST_REF_INIT) ref_init_tile<S>(f, p, T);
-    else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_tile<S>(f, p, T, arg);
-    else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, COMP>(f, p, T);
-    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, true, 1, true>(f, p, T, 0);
-    else if constexpr (STAGE == ST_CAN_ATROUS0_SQ) can_atrous_tile<S, true, 1, false>(f, p, T, 0);
-    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, false, 1, false>(f, p, T, arg);
-    else can_atrous_tile<S, false, 2, false>(f, p, T, arg);
+    else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_tile<S, L>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, COMP, L>(f, p, T);
+    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, true, 1, true, L>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS0_SQ) can_atrous_tile<S, true, 1, false, L>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, false, 1, false, L>(f, p, T, arg);
+    else can_atrous_tile<S, false, 2, false, L>(f, p, T, arg);
 }
 
 // Frame descriptor of a batch entry into scalar registers (the index is uniform across the block).
@@ -935,7 +982,8 @@ __device__ __forceinline__ void persist_tile(const FrameDesc* frames, uint32_t f
                                           uint32_t tile, int arg)
 {
     const FrameDesc f = load_frame(frames, fi);
-    run_stage<S, STAGE, COMP>(f, p, stage_tile<STAGE>(f.w, log2s, tile), arg);
+    constexpr bool CANON = STAGE != ST_REF_INIT && STAGE != ST_REF_ATROUS;
+    run_stage<S, STAGE, COMP, ArenaLds<CANON>>(f, p, stage_tile<STAGE>(f.w, log2s, tile), arg);
 }
 
 // Persistent form: every stage of a batch in one launch. Each workgroup takes chunks of
