@@ -974,9 +974,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchTable bt, KParams p, int a
     run_stage<S, STAGE, COMP>(f, p, T, arg);
 }
 
-// One tile of a persistent launch. Each stage is its own (non-inlined) function that loads the
-// frame descriptor itself: inlined into one kernel, the stages would keep every descriptor field
-// live across all of them and the register count would rise to that of all stages together.
+// One tile of a persistent launch. Each stage's branch loads the frame descriptor itself: one
+// load ahead of the branches would keep every descriptor field live across all of them and raise
+// the register count. The stage tiles share one LDS arena (ArenaLds).
 template <class S, int STAGE, bool COMP>
 __device__ __forceinline__ void persist_tile(const FrameDesc* frames, uint32_t fi, const KParams& p, uint32_t log2s,
                                           uint32_t tile, int arg)
