@@ -115,8 +115,8 @@ typedef struct svgf_desc {
                                     one launch per stage); read back with svgf_stage_times */
 #define SVGF_BATCH_PERSISTENT 0x4u /* run all stages of a batch in one persistent launch: the
                                       workgroups pull (stage, frame, tile) tickets in dependency
-                                      order and wait per frame for the previous stage, instead
-                                      of one launch per stage */
+                                      order and a tile waits until every tile of the previous
+                                      stage is done, instead of one launch per stage */
 
 typedef struct svgf_plane {
     void* ptr;            /* device pointer to row 0, pixel 0 */
