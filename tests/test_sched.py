@@ -31,3 +31,15 @@ def test_ticket_protocol(sim, seed, batches, workers):
     r = subprocess.run([str(sim), str(seed), str(batches), str(workers)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("ok ")
+
+
+def test_tiles_cover_every_pixel_once(tmp_path):
+    """Every stage's tiles (plain, residue pairs, tall) hold each pixel of a frame exactly once."""
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    exe = tmp_path / "tile_cover"
+    subprocess.run(["g++", "-O2", "-std=c++20", "-Wall", "-Wextra", "-Werror", "-o", str(exe),
+                    str(HERE / "tile_cover.cpp")], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("ok ")

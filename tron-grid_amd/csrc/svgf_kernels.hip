@@ -222,46 +222,8 @@ __host__ __device__ constexpr float log2k(int i, int j)
 }
 
 // ----------------------------------------------------------------------------------------
-// Tiles
+// Tiles (decoded by svgf_sched.hpp)
 // ----------------------------------------------------------------------------------------
-struct Tile {
-    int log2s, rx, ry, i0, j0;
-    __device__ int px(int i) const { return rx + (i << log2s); }
-    __device__ int py(int j) const { return ry + (j << log2s); }
-};
-
-__device__ __forceinline__ Tile decode_tile(int w, uint32_t log2s, uint32_t t)
-{
-    uint32_t s = 1u << log2s;
-    uint32_t lx = ((uint32_t)w + s - 1) >> log2s;
-    uint32_t ntx = (lx + LT - 1) / LT;
-    uint32_t res = t & (s * s - 1);
-    uint32_t tile = t >> (2 * log2s);
-    uint32_t ty = tile / ntx, tx = tile - ty * ntx;
-    return Tile{(int)log2s, (int)(res & (s - 1)), (int)(res >> log2s), (int)(tx * LT), (int)(ty * LT)};
-}
-
-// Tile t of a residue-pair stage (stride s >= 2): residues (rx, ry) and (rx + 1, ry), rx even,
-// ordered as decode_tile orders single residues.
-__device__ __forceinline__ Tile decode_pair_tile(int w, uint32_t log2s, uint32_t t)
-{
-    const uint32_t s = 1u << log2s;
-    const uint32_t lx = ((uint32_t)w + s - 1) >> log2s;
-    const uint32_t ntx = (lx + LT - 1) / LT;
-    const uint32_t res = t & (s * s / 2 - 1);
-    const uint32_t tile = t >> (2 * log2s - 1);
-    const uint32_t ty = tile / ntx, tx = tile - ty * ntx;
-    return Tile{(int)log2s, (int)(2 * (res & (s / 2 - 1))), (int)(res >> (log2s - 1)), (int)(tx * LT), (int)(ty * LT)};
-}
-
-// Tile t of the first CANONICAL a-trous iteration (stride 1): 16 x 32 pixels, row-major.
-__device__ __forceinline__ Tile decode_tall_tile(int w, uint32_t t)
-{
-    const uint32_t ntx = ((uint32_t)w + LT - 1) / LT;
-    const uint32_t ty = t / ntx, tx = t - ty * ntx;
-    return Tile{0, 0, 0, (int)(tx * LT), (int)(ty * 2 * LT)};
-}
-
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2). Renumber so
 // that XCD x runs one contiguous range of logical tiles: neighbouring tiles, which share
 // halo rows and (for s > 1) cache lines of other residues, then meet in the same L2.

@@ -72,6 +72,46 @@ SVGF_HD inline uint32_t stage_tiles(int canonical, int stage, uint32_t w, uint32
     return paired_stage(canonical, stage, log2s) ? pair_tiles_at(w, h, log2s) : tiles_at(w, h, log2s);
 }
 
+// A tile: lattice points (i0 + i, j0 + j) of residue (rx, ry) at stride 2^log2s (pixel
+// (rx + (i0 + i) 2^log2s, ry + (j0 + j) 2^log2s)); a pair tile also holds residue rx + 1.
+struct Tile {
+    int log2s, rx, ry, i0, j0;
+    SVGF_HD int px(int i) const { return rx + (i << log2s); }
+    SVGF_HD int py(int j) const { return ry + (j << log2s); }
+};
+
+SVGF_HD inline Tile decode_tile(int w, uint32_t log2s, uint32_t t)
+{
+    uint32_t s = 1u << log2s;
+    uint32_t lx = ((uint32_t)w + s - 1) >> log2s;
+    uint32_t ntx = (lx + LT - 1) / LT;
+    uint32_t res = t & (s * s - 1);
+    uint32_t tile = t >> (2 * log2s);
+    uint32_t ty = tile / ntx, tx = tile - ty * ntx;
+    return Tile{(int)log2s, (int)(res & (s - 1)), (int)(res >> log2s), (int)(tx * LT), (int)(ty * LT)};
+}
+
+// Tile t of a residue-pair stage (stride s >= 2): residues (rx, ry) and (rx + 1, ry), rx even,
+// ordered as decode_tile orders single residues.
+SVGF_HD inline Tile decode_pair_tile(int w, uint32_t log2s, uint32_t t)
+{
+    const uint32_t s = 1u << log2s;
+    const uint32_t lx = ((uint32_t)w + s - 1) >> log2s;
+    const uint32_t ntx = (lx + LT - 1) / LT;
+    const uint32_t res = t & (s * s / 2 - 1);
+    const uint32_t tile = t >> (2 * log2s - 1);
+    const uint32_t ty = tile / ntx, tx = tile - ty * ntx;
+    return Tile{(int)log2s, (int)(2 * (res & (s / 2 - 1))), (int)(res >> (log2s - 1)), (int)(tx * LT), (int)(ty * LT)};
+}
+
+// Tile t of the first CANONICAL a-trous iteration (stride 1): 16 x 32 pixels, row-major.
+SVGF_HD inline Tile decode_tall_tile(int w, uint32_t t)
+{
+    const uint32_t ntx = ((uint32_t)w + LT - 1) / LT;
+    const uint32_t ty = t / ntx, tx = t - ty * ntx;
+    return Tile{0, 0, 0, (int)(tx * LT), (int)(ty * 2 * LT)};
+}
+
 constexpr int MAX_STAGES = 12;
 constexpr uint32_t MAX_BATCH_FRAMES = 1024;
 constexpr uint32_t STARTS_ROW = MAX_BATCH_FRAMES + 1;
