@@ -82,7 +82,8 @@ typedef struct svgf_params {
     uint32_t iterations;  /* a-trous iterations, strides 1,2,4,...; reference 3 (main.cpp:929) */
     float phi_luma;       /* luminance edge-stop scale; reference SIGMA_COLOR_BASE = 2
                              (svgf_denoise.slang:137) */
-    float phi_normal;     /* CANONICAL: exponent of max(0, n.n'); 0 disables */
+    float phi_normal;     /* CANONICAL: exponent of max(0, n.n') for unit normals (n.n' is
+                             taken as at most 1); 0 disables */
     float phi_depth;      /* CANONICAL: depth edge-stop scale; 0 disables */
     float alpha_color;    /* CANONICAL: colour history blend */
     float alpha_moments;  /* moments blend; reference ALPHA_MOMENTS = 0.2 (svgf_denoise.slang:144) */
@@ -129,7 +130,7 @@ typedef struct svgf_plane {
  * Reference binding map (svgf_denoise.slang:61-94, main.cpp:2314-2342):
  *   radiance = binding 0 lighting_raw, denoised = binding 3, moments_prev = binding 4,
  *   moments_curr = binding 5. wavelet_a / wavelet_b (bindings 1, 2) are library scratch.
- * CANONICAL adds gbuffer (normal.xyz, linear depth; RGBA32F), gbuffer_prev, motion (RG32F,
+ * CANONICAL adds gbuffer (unit normal.xyz, linear depth; RGBA32F), gbuffer_prev, motion (RG32F,
  *   pixels, previous position = current position + motion) and the colour history
  *   (rgb, history length in alpha). */
 typedef struct svgf_frame {
