@@ -46,6 +46,7 @@ def progress(msg: str) -> None:
 #              the region it already holds in LDS (no extra plane); at stride 1 it also takes
 #              the depth gradient from the staged depths instead of the gradient plane
 def stage_bytes(storage_bytes4, storage_bytes2, iterations, batched):
+    """storage_bytes2: bytes of a moments pixel (8 for the wide FP16 state's four binary16 words)."""
     s4, s2, w4 = storage_bytes4, storage_bytes2, 16
     temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (w4 + s2 + 1 + 8) + (16 if batched else 0)
     atrous = [(w4 + 16) + (0 if k == 0 else 8) + (s4 if k == iterations - 1 else w4)
@@ -53,11 +54,46 @@ def stage_bytes(storage_bytes4, storage_bytes2, iterations, batched):
     return [temporal] + atrous
 
 
+# Stage-structured compulsory bytes per pixel of the survey's canonical FP16 layout (R, history,
+# moments, wavelets binary16; G and motion FP32; no gradient plane): temporal 72, variance 36,
+# 5 a-trous iterations of 32 (wavelet 8 + G 16 read, wavelet 8 written), iteration 0's history
+# write 8 = 276 per frame. The roofline reports its fraction with these bytes beside this layout's.
+SURVEY_FP16_ATROUS_BYTES_PX = 32
+SURVEY_FP16_FRAME_BYTES_PX = 276
+
+
 def _rel(a, b, floor=1e-3):
     import numpy as np
 
     r = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) / np.maximum(np.abs(b), floor)
     return float(r.max()), float(r.mean())
+
+
+def _state(ctx, hist, mom):
+    """(colour history (h, w, 3), moments (h, w, 2)) of a context's state planes as float64, the
+    wide FP16 state decoded."""
+    import numpy as np
+
+    import svgf_amd as S
+
+    h = hist.cpu().numpy()
+    m = mom.cpu().numpy()
+    if ctx.wide:
+        return S.decode_history(h)[0], S.decode_moments(m)
+    return h[..., :3].astype(np.float64), m.astype(np.float64)
+
+
+def _state_errors(den, hist3, mom2, ref, floor=1e-3):
+    """Per-output (max, mean) relative error against an oracle frame: denoised, colour history,
+    moments as the mean and the raw second moment mean^2 + sigma^2 (floor squared)."""
+    import numpy as np
+
+    m2g = mom2[..., 0] ** 2 + mom2[..., 1] ** 2
+    m2r = ref["moments"][..., 0] ** 2 + ref["moments"][..., 1] ** 2
+    return {"denoised": _rel(den[..., :3], ref["denoised"][..., :3], floor),
+            "history": _rel(hist3, ref["history"][..., :3], floor),
+            "moments_mean": _rel(mom2[..., 0], ref["moments"][..., 0], floor),
+            "moments_second": _rel(m2g, m2r, floor * floor)}
 
 
 def precision_report(frames: int, threads: int, scale: int) -> dict:
@@ -83,12 +119,12 @@ def precision_report(frames: int, threads: int, scale: int) -> dict:
         p = S.default_params(S.MODE_CANONICAL, storage)
         op = O.canonical_params(p.iterations, compensated=bool(p.flags & S.FLAG_COMPENSATED))
         hist = [torch.zeros((h, w, 4), dtype=dt, device=dev) for _ in range(2)]
-        mom = [torch.zeros((h, w, 2), dtype=dt, device=dev) for _ in range(2)]
         gb = [None, None]
         ohist = omom = ogp = None
-        worst = {k: [0.0, 0.0] for k in ("denoised", "history", "moments_mean")}
+        worst = {k: [0.0, 0.0] for k in ("denoised", "history", "moments_mean", "moments_second")}
         t_gpu = 0.0
         with S.Context(S.MODE_CANONICAL, storage, params=p, max_width=w, max_height=h) as ctx:
+            mom = [torch.zeros((h, w, ctx.moment_channels), dtype=dt, device=dev) for _ in range(2)]
             for f in range(frames):
                 d = S.synth_frame(w, h, seed=f, frame=f, noise=noise, pan_px=pan, disocclusion=0.1)
                 rad = d["radiance"]
@@ -113,14 +149,13 @@ def precision_report(frames: int, threads: int, scale: int) -> dict:
                 o = O.c_frame_f64(rad, d["gbuffer"], ogp, d["motion"], ohist, omom, op, history_valid=f > 0,
                                   threads=threads)
                 ohist, omom, ogp = o["history"], o["moments"], d["gbuffer"]
-                for key, g, r in (("denoised", den[..., :3], o["denoised"][..., :3]),
-                                  ("history", hist[cur][..., :3], o["history"][..., :3]),
-                                  ("moments_mean", mom[cur][..., 0], o["moments"][..., 0])):
-                    mx, mean = _rel(g.float().cpu().numpy(), r)
+                h3, m2 = _state(ctx, hist[cur], mom[cur])
+                for key, (mx, mean) in _state_errors(den.float().cpu().numpy(), h3, m2, o).items():
                     worst[key][0] = max(worst[key][0], mx)
                     worst[key][1] = max(worst[key][1], mean)
         out[name] = {"size": f"{w}x{h}", "frames": frames, "storage": "fp32" if storage == S.STORAGE_FP32 else "fp16",
                      "compensated_moments": bool(p.flags & S.FLAG_COMPENSATED),
+                     "wide_state": bool(p.flags & S.FLAG_WIDE_STATE),
                      "max_rel_err": {k: v[0] for k, v in worst.items()},
                      "mean_rel_err_worst_frame": {k: v[1] for k, v in worst.items()},
                      "rel_err_floor": 1e-3,
@@ -457,6 +492,8 @@ def main():
                     help="how the distinct frames repeat: forward and back (continuous), or in a loop (history "
                          "rejected at each restart)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP64-oracle check of the timed batched output")
     ap.add_argument("--agents", type=int, default=256, help="agent streams of the agent-frame tick (0: skip)")
     ap.add_argument("--agent-ticks", type=int, default=20)
     ap.add_argument("--composite-reps", type=int, default=50,
@@ -590,12 +627,18 @@ def main():
     def batch_step(ctx, k):
         S.check(L.svgf_denoise_batch(ctx.handle, reqs[k % P], args.frames, sh), "svgf_denoise_batch")
 
-    def time_batched(flags):
+    def time_batched(flags, inspect=None):
+        """Warmup, then the timed batches (events around each batch on the launch stream); with
+        SVGF_BATCH_TIMED_STAGES also the mean per-stage GPU time of the timed batches, from the
+        event nodes inside their graphs. `inspect(ctx)` runs after the timed region."""
         ctx = batched_context(flags)
+        stages = None
         with torch.cuda.stream(stream):
             for k in range(args.warmup):
                 batch_step(ctx, k)
             torch.cuda.synchronize()
+            if flags & S.BATCH_TIMED_STAGES:
+                ctx.stage_times()  # drops the warmup batches' times
             evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                    for _ in range(args.steps)]
             t0 = time.perf_counter()
@@ -605,20 +648,44 @@ def main():
                 evs[k][1].record(stream)
             torch.cuda.synchronize()
             t = time.perf_counter() - t0
+            if flags & S.BATCH_TIMED_STAGES:
+                stages = np.array(ctx.stage_times())
+        extra = inspect(ctx) if inspect else None
         ctx.close()
-        return t, np.array([a.elapsed_time(b) for a, b in evs])
+        return t, np.array([a.elapsed_time(b) for a, b in evs]), stages, extra
+
+    # accuracy of the timed output: the last timed batch's denoised frame and the state it left,
+    # for the first stream of every resolution class, against the FP64 oracle replaying that
+    # stream's whole sequence (below, outside the timed region)
+    sample_ids = sorted({wh: i for i, wh in reversed(list(enumerate(shapes)))}.values())
+
+    def capture(ctx):
+        got = {}
+        for i in sample_ids:
+            w, h = shapes[i]
+            st = ctx.stream_state(i)
+            hist = S.device_tensor_from_plane(st.history, h, w, 4, torch.float16)
+            mom = S.device_tensor_from_plane(st.moments, h, w, ctx.moment_channels, torch.float16)
+            h3, m2 = _state(ctx, hist, mom)
+            got[i] = (out16[i].float().cpu().numpy(), h3, m2)
+        return got
 
     forms = {"persistent": S.BATCH_PERSISTENT, "per_stage": 0}
     head = args.launch
-    t_batched, lat = time_batched(forms[head])
+    t_batched, lat, stage_ms, sampled = time_batched(forms[head] | S.BATCH_TIMED_STAGES, capture)
     batched_mpx = args.steps * batch_px / t_batched / 1e6
-    other = None
+    other = plain_graph = None
     if not args.batched_only:
         name = "per_stage" if head == "persistent" else "persistent"
-        t_o, lat_o = time_batched(forms[name])
+        t_o, lat_o, _, _ = time_batched(forms[name])
         other = {"form": name, "value": round(args.steps * batch_px / t_o / 1e6, 2), "unit": "Mpx/s",
                  "ms_per_step": round(t_o / args.steps * 1e3, 4),
                  "p99_ms": round(float(np.percentile(lat_o, 99)), 4)}
+        # the same form without the stage events in its graph (their cost)
+        t_n, lat_n, _, _ = time_batched(forms[head])
+        plain_graph = {"value": round(args.steps * batch_px / t_n / 1e6, 2), "unit": "Mpx/s",
+                       "ms_per_step": round(t_n / args.steps * 1e3, 4),
+                       "p99_ms": round(float(np.percentile(lat_n, 99)), 4)}
 
     if args.batched_only:
         print(json.dumps({"batched_fp16_mpx_per_s": round(batched_mpx, 2), "launch": head, "steps": args.steps,
@@ -666,27 +733,19 @@ def main():
         t_plain = time.perf_counter() - t0
     plain_mpx = args.steps * batch_px / t_plain / 1e6
 
-    progress("profiled pass")
-    # ---- profiled pass: per-stage kernel time (hipEvents around every stage launch) ----
-    ctxp = batched_context(S.PROFILE_STAGES)
-    stage_ms = []
-    with torch.cuda.stream(stream):
-        for k in range(args.warmup + args.steps):
-            batch_step(ctxp, k)
-            if k >= args.warmup:
-                stage_ms.append(ctxp.stage_times())
-    stage_ms = np.array(stage_ms).mean(axis=0)
     names = ["temporal"] + [f"atrous{k}" for k in range(p16.iterations)]
-    sb16 = stage_bytes(8, 4, p16.iterations, batched=True)
-    # Dominant kernel (largest share of the batch's GPU time): the a-trous kernel of the
-    # residue-pair iterations (strides 2..16: iterations 1..N-1, one code object launched
-    # once per iteration). Its roofline uses the mean algorithmic bytes per launch over the
-    # mean launch duration, the same average a per-kernel profiler summary reports for it.
-    first_pair = 1
-    at = slice(1 + first_pair, 1 + p16.iterations)
-    dom_ms = float(np.mean(stage_ms[at]))
-    dom_bytes_px = float(np.mean(sb16[at]))
-    achieved = dom_bytes_px * batch_px / (dom_ms * 1e-3) / 1e9
+    sb16 = stage_bytes(8, 8 if p16.flags & S.FLAG_WIDE_STATE else 4, p16.iterations, batched=True)
+    # Dominant kernel (largest share of the batch's GPU time): the residue-pair a-trous kernel of
+    # the iterations before the last (strides 2..2^(N-2): one code object, N-2 launches per
+    # batch; the last iteration is another kernel, which skips the filtered variance). `achieved`
+    # uses the survey's per-pixel bytes of an FP16 a-trous iteration (32 B/px) times the pixels
+    # of one launch, over the kernel's mean launch duration from the event nodes of the timed
+    # batches' own graphs; this layout's bytes (FP32 wavelets, gradient plane) are given beside.
+    at = slice(2, p16.iterations)
+    dom_ms = float(np.mean(stage_ms[at])) if stage_ms is not None else float("nan")
+    layout_bytes_px = float(np.mean(sb16[at]))
+    achieved = SURVEY_FP16_ATROUS_BYTES_PX * batch_px / (dom_ms * 1e-3) / 1e9
+    layout_achieved = layout_bytes_px * batch_px / (dom_ms * 1e-3) / 1e9
     bytes_px_16 = sum(sb16)
     bytes_px_32 = sum(stage_bytes(16, 8, p32.iterations, batched=False))
 
@@ -816,8 +875,41 @@ def main():
         tfile = ROOT / "profiles" / "traffic.txt"
         if tfile.exists() and args.frames == 64:
             for ln in tfile.read_text().splitlines():
-                if ln.startswith("atrous_pair_batched_fp16_mix64_bytes_per_launch:"):
+                if ln.startswith("atrous_pair_wide_batched_fp16_mix64_bytes_per_launch:"):
                     traffic = float(ln.split(":", 1)[1])
+
+    progress("accuracy of the timed batched output")
+    accuracy = None
+    if not args.no_accuracy:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+
+        threads = max(1, min(16, os.cpu_count() or 1))
+        op = O.canonical_params(p16.iterations, compensated=True)
+        K = args.warmup + args.steps
+        worst = {}
+        for i in sample_ids:
+            w, h = shapes[i]
+            hist = mom = gp = None
+            for q in range(K):
+                f, sgn = seq[q % P]
+                d = S.synth_frame(w, h, seed=1000 * i + f, stream=i, frame=f, pan_px=0.5, disocclusion=0.05)
+                rad = d["radiance"].astype(np.float16).astype(np.float32)  # the input the FP16 form receives
+                o = O.c_frame_f64(rad, d["gbuffer"], gp, d["motion"] if sgn > 0 else -d["motion"], hist, mom, op,
+                                  history_valid=q > 0, threads=threads)
+                hist, mom, gp = o["history"], o["moments"], d["gbuffer"]
+            den, h3, m2 = sampled[i]
+            for key, (mx, mean) in _state_errors(den, h3, m2, o).items():
+                a, b = worst.get(key, (0.0, 0.0))
+                worst[key] = (max(a, mx), max(b, mean))
+        accuracy = {"what": f"the last timed batch: denoised output, colour history and moments of the first "
+                            f"stream of each resolution class ({len(sample_ids)} streams, "
+                            f"{sum(shapes[i][0] * shapes[i][1] for i in sample_ids)} px) against the FP64 oracle "
+                            f"replaying each stream's {K} frames",
+                    "max_rel_err": {k: float(f"{v[0]:.4g}") for k, v in worst.items()},
+                    "mean_rel_err": {k: float(f"{v[1]:.4g}") for k, v in worst.items()},
+                    "rel_err_floor": 1e-3, "bound": 1e-3,
+                    "within_bound": all(v[0] <= 1e-3 for v in worst.values())}
 
     ms_per_step = t_batched / args.steps * 1e3
     line = {
@@ -846,9 +938,13 @@ def main():
             "iterations": p16.iterations,
             "params": {k: round(v, 6) if isinstance(v, float) else v for k, v in p16.as_dict().items()},
             "bytes_per_px": bytes_px_16,
-            "plane_layout": "radiance/history/wavelet (colour, sigma) rgba16f, moments (mean, sigma) rg16f, "
-                            "G normal.xyz+depth rgba32f, motion rg32f, history length u8, G copy rgba32f",
+            "plane_layout": "radiance rgba16f; history rgba16f (alpha: length + 3-bit rounding residual per "
+                            "channel); moments rgba16f (mean, sigma as binary16 hi + scaled lo words); wavelet "
+                            "(colour, sigma) rgba32f; G normal.xyz+depth rgba32f; motion rg32f; depth gradient "
+                            "rg32f; history length u8; G copy rgba32f",
+            "survey_bytes_per_px": SURVEY_FP16_FRAME_BYTES_PX,
         },
+        "accuracy": accuracy,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "what": "GPU time of one 64-frame batch"},
         "reference_mode_720p": ref720,
@@ -865,20 +961,32 @@ def main():
         "consumer_composite": composite,
         "producer_pack": pack,
         "batched_other_launch": other,
+        "batched_without_stage_events": plain_graph,
         "plain_fp32": {"value": round(plain_mpx, 2), "unit": "Mpx/s", "bytes_per_px": bytes_px_32,
                        "ratio_batched_over_plain": round(batched_mpx / plain_mpx, 3)},
         "effective_GBps": round(batched_mpx * 1e6 * bytes_px_16 / 1e9, 1),
-        "stage_ms": {n: round(float(t), 5) for n, t in zip(names, stage_ms)},
-        "roofline": {"bound": "hbm", "kernel": f"k_batch<F16S, ST_CAN_ATROUS_PAIR>: a-trous iterations {first_pair}..{p16.iterations - 1} "
-                                                 f"(strides {', '.join(str(1 << k) for k in range(first_pair, p16.iterations))}; "
-                                                 f"{p16.iterations - first_pair} launches per batch)",
+        "effective_GBps_survey_bytes": round(batched_mpx * 1e6 * SURVEY_FP16_FRAME_BYTES_PX / 1e9, 1),
+        "stage_ms": None if stage_ms is None else {n: round(float(t), 5) for n, t in zip(names, stage_ms)},
+        "roofline": {"bound": "hbm",
+                     "kernel": f"k_batch<F16S, ST_CAN_ATROUS_PAIR, SF_WIDE>: a-trous iterations 1..{p16.iterations - 2} "
+                               f"(strides {', '.join(str(1 << k) for k in range(1, p16.iterations - 1))}; "
+                               f"{p16.iterations - 2} launches per batch)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": round(dom_bytes_px * batch_px),
-                     "algorithmic_bytes_per_px": dom_bytes_px, "mean_launch_ms": round(dom_ms, 5),
-                     "launch_timing": "hipEvents around each stage launch on the launch stream, in a pass over the "
-                                      "same batches after the timed one (per-stage events need launches outside "
-                                      "the graph); mean of the kernel's launches per batch",
+                     "traffic_source": (None if traffic is None else
+                                        "profiles/traffic.txt: rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per "
+                                        "launch of this kernel on this workload (committed figure, not measured "
+                                        "in this run)" if args.traffic is None else "--traffic argument"),
+                     "algorithmic_bytes_per_px": SURVEY_FP16_ATROUS_BYTES_PX,
+                     "algorithmic_bytes_per_launch": SURVEY_FP16_ATROUS_BYTES_PX * batch_px,
+                     "pixels_per_launch": batch_px,
+                     "layout_bytes_per_px": layout_bytes_px,
+                     "layout_achieved": round(layout_achieved, 1),
+                     "layout_frac": round(layout_achieved / HBM_PEAK_GBPS, 4),
+                     "mean_launch_ms": round(dom_ms, 5),
+                     "launch_timing": "hipEvents recorded by event nodes around each stage launch inside the "
+                                      "graphs of the timed batches (SVGF_BATCH_TIMED_STAGES), on the launch "
+                                      "stream; mean over the kernel's launches in the timed region",
                      "note": "arithmetic-bound: ~16 VALU instructions (2 transcendental) per tap, 25 taps per "
                              "pixel, with the VALU and LDS the busiest units; HBM fraction is not the binding limit"},
         "cpu_baseline": cpu,
@@ -886,7 +994,6 @@ def main():
     }
     print(json.dumps(line))
     ctx32.close()
-    ctxp.close()
 
 
 if __name__ == "__main__":

@@ -77,6 +77,16 @@ typedef enum svgf_format {
 #define SVGF_FLAG_COMPENSATED_MOMENTS 0x1u /* moments kept as (mean, sqrt(centred 2nd moment)),
                                               updated by the West/Finch recursion; never forms
                                               E[x^2]. Default for FP16 storage. */
+#define SVGF_FLAG_WIDE_STATE 0x2u /* CANONICAL FP16 with compensated moments only (default
+    there): the fed-back state carries its binary16 rounding residual, so the rounding does not
+    compound from frame to frame (history_cap <= 63).
+      history planes, RGBA16F: rgb = binary16 of the colour; the alpha slot is a 16-bit word, not
+        a half: bits 0-5 history length, bits 6-8 / 9-11 / 12-14 residual code q of r / g / b
+        (two's complement -4..3); a channel with binary16 value h of biased exponent e decodes as
+        h + q 2^(max(e, 1) - 28).
+      moments planes, RGBA16F (not RG16F): (mean_hi, sigma_hi, mean_lo, sigma_lo); a value
+        decodes as hi + lo 2^(max(e, 1) - 25), e the biased exponent of hi.
+    Without the flag the FP16 history alpha is the length as a half and the moments are RG16F. */
 
 typedef struct svgf_params {
     uint32_t iterations;  /* a-trous iterations, strides 1,2,4,...; reference 3 (main.cpp:929) */
@@ -118,6 +128,12 @@ typedef struct svgf_desc {
                                       workgroups pull (stage, frame, tile) tickets in dependency
                                       order and a tile waits until every tile of the previous
                                       stage is done, instead of one launch per stage */
+#define SVGF_BATCH_TIMED_STAGES 0x8u /* batched form: record a hipEvent before and after every
+                                        stage launch of each batch, inside the captured graph
+                                        (event nodes), so that per-stage GPU times come from the
+                                        batches themselves; svgf_stage_times then returns their
+                                        mean over the batches completed since its previous call
+                                        (the persistent form: one stage, the whole launch) */
 
 typedef struct svgf_plane {
     void* ptr;            /* device pointer to row 0, pixel 0 */
@@ -179,7 +195,8 @@ typedef struct svgf_context svgf_context;
 SVGF_API uint32_t svgf_abi_version(void);
 SVGF_API const char* svgf_status_str(svgf_status status);
 /* Defaults: REFERENCE = the shader's constants (3 iterations, phi 2, alpha 0.2, eps 1e-4);
- * CANONICAL = 5 iterations, phi_luma 4, phi_normal 128, phi_depth 1, alpha 0.2/0.2. */
+ * CANONICAL = 5 iterations, phi_luma 4, phi_normal 128, phi_depth 1, alpha 0.2/0.2, history
+ * cap 32; with FP16 storage also SVGF_FLAG_COMPENSATED_MOMENTS | SVGF_FLAG_WIDE_STATE. */
 SVGF_API svgf_status svgf_default_params(uint32_t mode, uint32_t storage, svgf_params* out);
 /* Bytes the library allocates: scratch for a context (at max_width x max_height and the
  * batch limits) and the resident state of one stream of w x h. */
@@ -203,19 +220,33 @@ SVGF_API svgf_status svgf_stream_state_get(svgf_context* ctx, uint32_t stream_id
 SVGF_API svgf_status svgf_budget_used(svgf_context* ctx, uint64_t* bytes);
 
 /* --- batched form: n requests of uneven sizes in one ragged-tile launch sequence -------- *
+ * Each call that returns SVGF_OK enqueues one batch, numbered 1, 2, ... (svgf_batch_last).
  * With SVGF_BATCH_PERSISTENT, a batch is one kernel launch; its dependency waits are bounded
  * (250 ms each; the environment variable SVGF_PERSISTENT_WAIT_US, read by svgf_create, sets
- * another bound). A batch whose wait gave up has wrong outputs: the next call that reuses its
- * internal slot (every third call) returns SVGF_ERR_TIMEOUT and enqueues nothing. */
+ * another bound). A batch whose wait gave up has wrong outputs; once its completion is
+ * observed (by svgf_batch_status, or by a later svgf_denoise_batch / svgf_stream_close call)
+ * its streams restart with cold history, so no later frame reprojects the wrong state. In the
+ * persistent form a call whose requests name a stream of a batch still in flight first waits
+ * for that batch on the host. */
 SVGF_API svgf_status svgf_denoise_batch(svgf_context* ctx, const svgf_request* requests, uint32_t n,
                                void* hip_stream);
+/* Number of the batch the last successful svgf_denoise_batch call enqueued (0: none yet). */
+SVGF_API svgf_status svgf_batch_last(svgf_context* ctx, uint64_t* batch);
+/* Outcome of a batch: SVGF_OK when it completed correctly, SVGF_ERR_TIMEOUT when a dependency
+ * wait of its persistent launch gave up (its outputs are wrong; its streams have been reset to
+ * cold history), SVGF_ERR_NOT_READY while it runs (wait = 0; wait = 1 blocks until it has
+ * completed). The outcomes of the last 3 batches are kept; an older or unknown number is
+ * SVGF_ERR_INVALID_ARGUMENT. */
+SVGF_API svgf_status svgf_batch_status(svgf_context* ctx, uint64_t batch, int32_t wait);
 
-/* --- per-stage GPU time of the last svgf_denoise / svgf_denoise_batch call --------------
- * Needs SVGF_PROFILE_STAGES in svgf_desc.batch_flags (hipEvents around each stage launch, the
- * counterpart of the reference's timestamp pair around the SVGF chain, main.cpp:907, 947).
- * Waits for that call to finish. Stage order: REFERENCE init, a-trous 0..N-1; CANONICAL
- * temporal, a-trous 0..N-1 (a-trous 0 also computes the spatial variance of young pixels).
- * *count receives the number of stages. */
+/* --- per-stage GPU time ---------------------------------------------------------------
+ * With SVGF_PROFILE_STAGES in svgf_desc.batch_flags: of the last svgf_denoise /
+ * svgf_denoise_batch call (hipEvents around each stage launch, the counterpart of the
+ * reference's timestamp pair around the SVGF chain, main.cpp:907, 947); waits for that call to
+ * finish. With SVGF_BATCH_TIMED_STAGES (and not SVGF_PROFILE_STAGES): the mean over the batches
+ * completed since the previous call, waiting for those in flight (SVGF_ERR_NOT_READY if none).
+ * Stage order: REFERENCE init, a-trous 0..N-1; CANONICAL temporal, a-trous 0..N-1 (a-trous 0
+ * also computes the spatial variance of young pixels). *count receives the number of stages. */
 SVGF_API svgf_status svgf_stage_times(svgf_context* ctx, float* ms, uint32_t capacity, uint32_t* count);
 
 /* --- serving form: continuous batching over resident streams -----------------------------

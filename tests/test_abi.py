@@ -44,7 +44,7 @@ def test_default_params_reference_constants():
     assert p.variance_eps == pytest.approx(1e-4)
     assert p.flags == 0
     c16 = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
-    assert c16.iterations == 5 and c16.flags & S.FLAG_COMPENSATED
+    assert c16.iterations == 5 and c16.flags == S.FLAG_COMPENSATED | S.FLAG_WIDE_STATE
     assert S.default_params(S.MODE_CANONICAL, S.STORAGE_FP32).flags == 0
 
 
@@ -59,9 +59,14 @@ def test_query_bytes():
     ctx_b, st_b = ctypes.c_uint64(), ctypes.c_uint64()
     assert S.lib().svgf_query_bytes(ctypes.byref(d), 1280, 720, ctypes.byref(ctx_b), ctypes.byref(st_b)) == 0
     px = 1280 * 720
-    # history 2 x rgba16f + moments 2 x rg16f + G copy 2 x rgba32f
-    assert st_b.value >= px * (2 * 8 + 2 * 4 + 2 * 16)
-    assert st_b.value < px * (2 * 8 + 2 * 4 + 2 * 16) + 6 * 256
+    # default FP16 form (wide state): history 2 x rgba16f + moments 2 x rgba16f (hi / lo words)
+    # + G copy 2 x rgba32f
+    assert st_b.value >= px * (2 * 8 + 2 * 8 + 2 * 16)
+    assert st_b.value < px * (2 * 8 + 2 * 8 + 2 * 16) + 6 * 256
+    # without the wide state: moments 2 x rg16f
+    d.params.flags = S.FLAG_COMPENSATED
+    assert S.lib().svgf_query_bytes(ctypes.byref(d), 1280, 720, None, ctypes.byref(st_b)) == 0
+    assert px * (2 * 8 + 2 * 4 + 2 * 16) <= st_b.value < px * (2 * 8 + 2 * 4 + 2 * 16) + 6 * 256
     d.mode = S.MODE_REFERENCE
     d.storage = S.STORAGE_FP32
     assert S.lib().svgf_query_bytes(ctypes.byref(d), 640, 360, None, ctypes.byref(st_b)) == 0
@@ -90,7 +95,7 @@ def test_create_rejects_bad_descriptors():
     bad = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, S.default_params(0), 64, 64, 0, 0, 0, 0, 0, 0x80)
     assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == 1
     # every declared batch flag passes validation (here it then stops at the device check)
-    known = S.BATCH_NO_GRAPH | S.PROFILE_STAGES | S.BATCH_PERSISTENT
+    known = S.BATCH_NO_GRAPH | S.PROFILE_STAGES | S.BATCH_PERSISTENT | S.BATCH_TIMED_STAGES
     ok = S.Desc(S.MODE_CANONICAL, S.STORAGE_FP16, S.default_params(1, 1), 64, 64, 4, 4, 4096, 0, 0, known)
     try:
         import torch
@@ -111,6 +116,14 @@ def test_create_rejects_bad_descriptors():
     p.flags = S.FLAG_COMPENSATED  # compensated moments are not the reference's semantics
     bad = S.Desc(S.MODE_REFERENCE, S.STORAGE_FP32, p, 64, 64, 0, 0, 0, 0, 0, 0)
     assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == 2
+    # the wide state: FP16 CANONICAL with compensated moments only, lengths up to 63
+    for storage, flags, cap, want in ((S.STORAGE_FP32, S.FLAG_COMPENSATED | S.FLAG_WIDE_STATE, 32, 2),
+                                      (S.STORAGE_FP16, S.FLAG_WIDE_STATE, 32, 2),
+                                      (S.STORAGE_FP16, S.FLAG_COMPENSATED | S.FLAG_WIDE_STATE, 64, 1)):
+        p = S.default_params(S.MODE_CANONICAL, storage)
+        p.flags, p.history_cap = flags, cap
+        bad = S.Desc(S.MODE_CANONICAL, storage, p, 64, 64, 0, 0, 0, 0, 0, 0)
+        assert L.svgf_create(ctypes.byref(bad), ctypes.byref(h)) == want, (storage, flags, cap)
     assert L.svgf_create(None, ctypes.byref(h)) == 1
     assert L.svgf_destroy(None) == 1
 
@@ -188,6 +201,8 @@ def test_binding_flags_match_header():
     assert defs["PROFILE_STAGES"] == S.PROFILE_STAGES
     assert defs["BATCH_PERSISTENT"] == S.BATCH_PERSISTENT
     assert defs["FLAG_COMPENSATED_MOMENTS"] == S.FLAG_COMPENSATED
+    assert defs["FLAG_WIDE_STATE"] == S.FLAG_WIDE_STATE
+    assert defs["BATCH_TIMED_STAGES"] == S.BATCH_TIMED_STAGES
 
 
 def test_wire_and_readback_argument_checks_without_gpu():

@@ -9,7 +9,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from svgf_testutil import gpu_canonical_sequence, oracle_canonical_sequence, rel_err, seq_inputs
+from svgf_testutil import gpu_canonical_sequence, moment_errors, oracle_canonical_sequence, rel_err, seq_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -131,20 +131,41 @@ def test_canonical_edge_stop_settings(S, oracle_lib, phis):
         assert dm <= 2e-4 and dmean <= 1e-6, f
 
 
-def test_canonical_fp16_compensated_sequence(S, oracle_lib):
-    """Precision form: FP16 storage, compensated moments, lognormal noise, vs FP64 oracle."""
+@pytest.mark.parametrize("wide", [True, False], ids=["wide_state", "plain_fp16_state"])
+def test_canonical_fp16_compensated_sequence(S, oracle_lib, wide):
+    """Precision form: FP16 storage, compensated moments, lognormal noise, 16 frames, vs the
+    FP64 oracle. With the wide state (the default) every output stays within 1e-3 MAX relative
+    error at every frame: denoised, colour history, moments (mean and raw second moment).
+    Without it the binary16 roundings of the fed-back state compound (about u / alpha = 5
+    half-ulps), which is why the wide state exists: the mean error stays small but the max is
+    only bounded by 4e-3."""
     O = oracle_lib
-    w, h = 200, 120
-    inputs = seq_inputs(w, h, 6, noise=1, pan_px=1.37, disocclusion=0.1, half=True)
+    w, h = 256, 160
+    inputs = seq_inputs(w, h, 16, noise=1, pan_px=1.37, disocclusion=0.1, half=True)
     op = O.canonical_params(5, compensated=True)
-    ref = oracle_canonical_sequence(inputs, op)
-    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=w, max_height=h) as ctx:
+    ref = oracle_canonical_sequence(inputs, op, threads=8)
+    p = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
+    assert p.flags & S.FLAG_WIDE_STATE
+    if not wide:
+        p.flags = S.FLAG_COMPENSATED
+    bound = 1e-3 if wide else 4e-3
+    mean_bound = 3e-4 if wide else 1e-3
+    worst = {}
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p, max_width=w, max_height=h) as ctx:
+        assert ctx.wide == wide and ctx.moment_channels == (4 if wide else 2)
         got = gpu_canonical_sequence(ctx, inputs, torch.float16)
     for f, (g, r) in enumerate(zip(got, ref)):
-        dm, dmean = rel_err(g["denoised"][..., :3], r["denoised"][..., :3])
-        print(f"frame {f}: denoised max {dm:.3e} mean {dmean:.3e}")
+        errs = {"denoised": rel_err(g["denoised"][..., :3], r["denoised"][..., :3]),
+                "history": rel_err(g["history"][..., :3], r["history"][..., :3])}
+        me = moment_errors(g["moments"], r["moments"])
+        errs["moments_mean"], errs["moments_second"] = me["mean"], me["second"]
         assert np.isfinite(g["denoised"]).all()
-        assert dmean <= 1e-3, f
+        assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), f
+        for n, (mx, mean) in errs.items():
+            worst[n] = max(worst.get(n, 0.0), mx)
+            # (the binary16 rounding of the output alone averages ~1.8e-4 relative)
+            assert mx <= bound and mean <= mean_bound, (f, n, mx, mean)
+    print(f"wide={wide}: max rel err over 16 frames", {n: f"{v:.3e}" for n, v in worst.items()})
 
 
 def _ragged_shapes():
@@ -268,40 +289,86 @@ def test_invalid_arguments(S):
 def test_persistent_wait_limit_reports_timeout(S, monkeypatch):
     """The persistent form's bounded dependency wait, forced from the host: with a 0 us limit
     every wait that finds its dependency unfinished gives up at once (the kernel still ends
-    normally), and the next call that reuses the slot returns SVGF_ERR_TIMEOUT. A context with
-    the default limit on the same inputs is unaffected. One 1280x720 frame per batch: its
-    stage i + 1 tickets directly follow its stage i tickets, so workgroups do wait (in a batch of
-    many frames a frame's previous stage has long finished when its next stage's tickets come)."""
+    normally). svgf_batch_status then reports SVGF_ERR_TIMEOUT for that very batch, and its
+    stream restarts with cold history (frames = 0), so the next batch does not reproject the
+    wrong state; a batch that completed normally reports SVGF_OK and advances the stream. A
+    context with the default limit on the same inputs reports SVGF_OK for every batch. One
+    1280x720 frame per batch: its stage i + 1 tickets directly follow its stage i tickets, so
+    workgroups do wait (in a batch of many frames a frame's previous stage has long finished
+    when its next stage's tickets come)."""
     dev = torch.device("cuda")
     w, h = 1280, 720
     d = S.synth_frame(w, h, seed=5, frame=0)
     rad = torch.from_numpy(d["radiance"]).to(dev).half()
     g = torch.from_numpy(d["gbuffer"]).to(dev)
     mv = torch.from_numpy(d["motion"]).to(dev)
-    n = 1
 
     def run(ctx, calls):
-        sids = [ctx.stream_open(w, h) for _ in range(n)]
-        outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for _ in range(n)]
+        sid = ctx.stream_open(w, h)
+        out = torch.empty((h, w, 4), dtype=torch.float16, device=dev)
         statuses = []
-        for _ in range(calls):
-            items = [{"stream_id": s, "radiance": rad, "gbuffer": g, "motion": mv, "denoised": o}
-                     for s, o in zip(sids, outs)]
-            try:
-                ctx.denoise_batch(items)
-                statuses.append(0)
-            except S.SvgfError as e:
-                statuses.append(e.status)
-        torch.cuda.synchronize()
+        for k in range(calls):
+            before = ctx.stream_state(sid).frames
+            ctx.denoise_batch([{"stream_id": sid, "radiance": rad, "gbuffer": g, "motion": mv, "denoised": out}])
+            b = ctx.batch_last()
+            assert b == k + 1
+            st = ctx.batch_status(b, wait=True)
+            statuses.append(st)
+            frames = ctx.stream_state(sid).frames
+            assert frames == (0 if st == 9 else before + 1), (k, st, before, frames)
+        assert ctx.batch_status(calls + 5) == 1  # never enqueued
         return statuses
 
-    kw = dict(max_width=w, max_height=h, max_streams=n, max_batch_frames=n, max_batch_pixels=n * w * h,
+    kw = dict(max_width=w, max_height=h, max_streams=1, max_batch_frames=1, max_batch_pixels=w * h,
               batch_flags=S.BATCH_PERSISTENT)
     monkeypatch.setenv("SVGF_PERSISTENT_WAIT_US", "0")
     with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, **kw) as ctx:
         st = run(ctx, 6)
     monkeypatch.delenv("SVGF_PERSISTENT_WAIT_US")
     print("statuses with a 0 us wait limit:", st)
-    assert st[:3] == [0, 0, 0] and 9 in st[3:]
+    assert set(st) <= {0, 9} and 9 in st
     with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, **kw) as ctx:
         assert run(ctx, 6) == [0] * 6
+
+
+@pytest.mark.parametrize("persistent", [False, True], ids=["per_stage", "persistent"])
+def test_timed_stages_inside_graph(S, persistent):
+    """SVGF_BATCH_TIMED_STAGES puts event-record nodes around the stage kernels of each batch's
+    graph: svgf_stage_times returns positive per-stage times averaged over the completed
+    batches (one time for the persistent launch), and the outputs are those of the same batches
+    without the events."""
+    dev = torch.device("cuda")
+    shapes = [(96, 96), (320, 180), (640, 480)]
+    npx = sum(w * h for w, h in shapes)
+    inputs = [seq_inputs(w, h, 4, seed0=20 * i, stream=i, pan_px=0.6, disocclusion=0.05) for i, (w, h) in
+              enumerate(shapes)]
+    base = S.BATCH_PERSISTENT if persistent else 0
+    res = {}
+    for flags in (base, base | S.BATCH_TIMED_STAGES):
+        with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=640, max_height=480, max_streams=3,
+                       max_batch_frames=3, max_batch_pixels=npx, batch_flags=flags) as ctx:
+            sids = [ctx.stream_open(w, h) for w, h in shapes]
+            outs = []
+            for f in range(4):
+                items = []
+                for i, (w, h) in enumerate(shapes):
+                    d = inputs[i][f]
+                    items.append({"stream_id": sids[i], "radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                                  "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                                  "motion": torch.from_numpy(d["motion"]).to(dev),
+                                  "denoised": torch.empty((h, w, 4), dtype=torch.float16, device=dev)})
+                ctx.denoise_batch(items)
+                outs.append([it["denoised"] for it in items])
+            torch.cuda.synchronize()
+            if flags & S.BATCH_TIMED_STAGES:
+                t = ctx.stage_times()
+                assert len(t) == (1 if persistent else ctx.params.iterations + 1), t
+                assert all(0.0 < x < 100.0 for x in t), t
+                with pytest.raises(S.SvgfError) as e:  # nothing completed since
+                    ctx.stage_times()
+                assert e.value.status == 10
+            res[flags] = [[o.view(torch.int16).cpu() for o in row] for row in outs]
+    a, b = res.values()
+    for f in range(4):
+        for i in range(len(shapes)):
+            assert torch.equal(a[f][i], b[f][i]), (f, i)

@@ -6,6 +6,7 @@ single-frame form.
 """
 from __future__ import annotations
 
+import ctypes
 import random
 import time
 
@@ -127,7 +128,12 @@ def test_simulate_admission_budget(S):
     d = S.synth_frame(128, 128, seed=0, frame=0)
     ins = [{"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
             "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev), "motion": torch.from_numpy(d["motion"]).to(dev)}]
-    per_stream = 128 * 128 * (2 * 8 + 2 * 4 + 2 * 16)
+    desc = S.Desc(S.MODE_CANONICAL, S.STORAGE_FP16, S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16), 128, 128,
+                  0, 0, 0, 0, 0, 0)
+    sb = ctypes.c_uint64()
+    S.check(S.lib().svgf_query_bytes(ctypes.byref(desc), 128, 128, None, ctypes.byref(sb)), "svgf_query_bytes")
+    per_stream = sb.value  # history 2 x rgba16f + moments 2 x rgba16f (wide state) + G copy 2 x rgba32f
+    assert per_stream >= 128 * 128 * (2 * 8 + 2 * 8 + 2 * 16)
     with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=128, max_height=128, max_streams=16,
                    max_batch_frames=8, max_batch_pixels=8 * 128 * 128, hbm_budget_bytes=4 * per_stream + 4096) as ctx:
         with S.Server(ctx, lanes=1) as srv:

@@ -1,4 +1,5 @@
-"""Structure of the generated gfx950 code of the persistent batched kernel (CPU only: hipcc -S).
+"""Structure of the generated gfx950 code (CPU only: hipcc -S): the persistent batched kernel's
+barriers, register spills, and the absence of packed FP32 instructions.
 
 Regression test for a hang: written as two `tid == 0` blocks on either side of the ticket
 loop's back edge (publish a tile; take the next ticket), the loop was restructured by the
@@ -58,7 +59,8 @@ def _loop_of_blocks(lines):
 
 def test_persistent_ticket_loop_barriers_are_not_divergent(asm):
     kernels = list(_functions(asm, r"_ZN4svgf12_GLOBAL__N_19k_persist"))
-    assert len(kernels) == 6  # REFERENCE fp32/fp16, CANONICAL fp32/fp16 x raw/compensated moments
+    # REFERENCE fp32/fp16, CANONICAL fp32/fp16 x raw/compensated moments, fp16 wide state
+    assert len(kernels) == 7
     for name, lines in kernels:
         loops = _loop_of_blocks(lines)
         latches = {loops[n][0] for n, l in enumerate(lines) if "s_andn2_b64 exec, exec" in l and loops[n]}
@@ -78,3 +80,35 @@ def test_hot_kernels_do_not_spill(asm):
         name = re.search(r"\.name:\s+(\S+)", blk).group(1)
         scratch = int(re.search(r"\.private_segment_fixed_size:\s+(\d+)", blk).group(1))
         assert scratch == 0, name
+
+
+# the library's device flags (tron-grid_amd/Makefile)
+DEVICE_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
+                "-packed-fp32-ops", "--cuda-device-only", "-S"]
+PACKED_F32 = re.compile(r"\bv_pk_(add|mul|fma)_f32\b")
+
+
+def _device_asm(src, out, flags):
+    hipcc = HIPCC if pathlib.Path(HIPCC).exists() else "hipcc"
+    lang = ["-x", "hip"] if src.suffix == ".cpp" else []
+    subprocess.run([hipcc] + flags + lang + ["-o", str(out), str(src)], check=True, capture_output=True)
+    return out.read_text()
+
+
+def test_no_packed_fp32_instructions(asm, tmp_path):
+    """Every HIP source of the library, built with the library's flags, contains no
+    v_pk_add_f32 / v_pk_mul_f32 / v_pk_fma_f32 (the -packed-fp32-ops target feature applies to
+    the device pass; the warning hipcc prints about it comes from the host pass). Control: the
+    kernels built without the feature do contain them, so the check can fail."""
+    if not pathlib.Path(HIPCC).exists() and not shutil.which("hipcc"):
+        pytest.skip("hipcc not available")
+    assert not PACKED_F32.search(asm)
+    csrc = ROOT / "tron-grid_amd" / "csrc"
+    for src in sorted(csrc.glob("*.hip")) + sorted(csrc.glob("*.cpp")):
+        if src.name == "svgf_kernels.hip":
+            continue
+        text = _device_asm(src, tmp_path / (src.name + ".s"), DEVICE_FLAGS)
+        assert not PACKED_F32.search(text), src.name
+    flags = [f for f in DEVICE_FLAGS if f not in ("-Xclang", "-target-feature", "-packed-fp32-ops")]
+    control = _device_asm(csrc / "svgf_kernels.hip", tmp_path / "control.s", flags)
+    assert PACKED_F32.search(control)

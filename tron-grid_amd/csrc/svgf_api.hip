@@ -59,6 +59,11 @@ struct BatchSlot {
     PersistCtl* ctl = nullptr;  // persistent form: ticket and completion counters (zero between launches)
     uint32_t* err_host = nullptr;  // persistent form: host-mapped timeout flag
     uint32_t* err_dev = nullptr;
+    hipEvent_t sev[MAX_STAGES + 1]{};  // SVGF_BATCH_TIMED_STAGES: events around the batch's stage launches
+    int sev_n = 0;                      // stages those events bracket in the slot's last batch
+    uint64_t seq = 0;        // id of the batch the slot last ran (0: none)
+    bool checked = false;    // its outcome has been read (after completion)
+    bool failed = false;     // a dependency wait of its persistent launch gave up
 };
 
 }  // namespace
@@ -82,20 +87,26 @@ struct svgf_context {
 
     BatchSlot slots[BATCH_SLOTS];
     int next_slot = 0;
+    uint64_t last_seq = 0;  // id of the last batch enqueued
     size_t table_bytes = 0, starts_off = 0, frames_off = 0, arena_bytes = 0;
     int grid_blocks = 0;
     uint32_t persist_blocks = 0;  // resident workgroups of the persistent kernel on the device
+    int cus = 1;                  // compute units of the device
     uint64_t max_polls = 0;       // persistent form: longest dependency wait, in polls
     hipStream_t capture_stream = nullptr;
 
     // SVGF_PROFILE_STAGES
     hipEvent_t ev[MAX_STAGES + 1]{};
     int ev_stages = 0;
+    // SVGF_BATCH_TIMED_STAGES: per-stage GPU time summed over completed batches
+    double stage_sum[MAX_STAGES]{};
+    int stage_n = 0;
+    uint64_t stage_batches = 0;
 };
 
 namespace {
 
-svgf_status validate_params(const svgf_params& p, uint32_t mode)
+svgf_status validate_params(const svgf_params& p, uint32_t mode, uint32_t storage)
 {
     if (p.iterations < 1 || p.iterations > 10) return SVGF_ERR_INVALID_ARGUMENT;
     auto finite_nonneg = [](float v) { return v >= 0.0f && v < 3.0e38f; };
@@ -107,8 +118,13 @@ svgf_status validate_params(const svgf_params& p, uint32_t mode)
             !(p.normal_reject >= -1.0f && p.normal_reject <= 1.0f) || p.history_cap < 1 || p.history_cap > 255)
             return SVGF_ERR_INVALID_ARGUMENT;
     }
-    if (p.flags & ~SVGF_FLAG_COMPENSATED_MOMENTS) return SVGF_ERR_INVALID_ARGUMENT;
-    if (mode == SVGF_MODE_REFERENCE && (p.flags & SVGF_FLAG_COMPENSATED_MOMENTS)) return SVGF_ERR_UNSUPPORTED;
+    if (p.flags & ~(SVGF_FLAG_COMPENSATED_MOMENTS | SVGF_FLAG_WIDE_STATE)) return SVGF_ERR_INVALID_ARGUMENT;
+    if (mode == SVGF_MODE_REFERENCE && p.flags) return SVGF_ERR_UNSUPPORTED;
+    if (p.flags & SVGF_FLAG_WIDE_STATE) {
+        // binary16 words of compensated moments; the length field has 6 bits
+        if (storage != SVGF_STORAGE_FP16 || !(p.flags & SVGF_FLAG_COMPENSATED_MOMENTS)) return SVGF_ERR_UNSUPPORTED;
+        if (p.history_cap > WIDE_MAX_LEN) return SVGF_ERR_INVALID_ARGUMENT;
+    }
     return SVGF_OK;
 }
 
@@ -127,6 +143,7 @@ KParams to_kparams(const svgf_params& p)
     k.history_cap = (int32_t)p.history_cap;
     k.iterations = (int32_t)p.iterations;
     k.compensated = (p.flags & SVGF_FLAG_COMPENSATED_MOMENTS) ? 1 : 0;
+    k.wide = (p.flags & SVGF_FLAG_WIDE_STATE) ? 1 : 0;
     constexpr float LOG2E = 1.44269504f, LN2 = 0.693147181f;
     k.luma_k = p.phi_luma > 0.0f ? -LOG2E / p.phi_luma : 0.0f;
     k.depth_c = LN2 * p.phi_depth;
@@ -147,7 +164,18 @@ svgf_status check_plane(const svgf_plane& pl, uint32_t want_fmt, uint32_t w)
 PlaneDev dev_plane(const svgf_plane& p) { return PlaneDev{static_cast<char*>(p.ptr), p.pitch_bytes}; }
 
 uint32_t fmt4(const svgf_context* c) { return c->st == Storage::F32 ? SVGF_FMT_RGBA32F : SVGF_FMT_RGBA16F; }
-uint32_t fmt2(const svgf_context* c) { return c->st == Storage::F32 ? SVGF_FMT_RG32F : SVGF_FMT_RG16F; }
+// moments: two channels at the storage width, or the wide state's four binary16 words
+uint32_t fmt2(const svgf_context* c)
+{
+    if (c->kp.wide) return SVGF_FMT_RGBA16F;
+    return c->st == Storage::F32 ? SVGF_FMT_RG32F : SVGF_FMT_RG16F;
+}
+// bytes of a moments pixel
+uint32_t moment_bytes(uint32_t storage, uint32_t flags)
+{
+    if (flags & SVGF_FLAG_WIDE_STATE) return 8;
+    return storage == SVGF_STORAGE_FP32 ? 8 : 4;
+}
 
 uint64_t stream_state_bytes(uint32_t mode, uint32_t s4, uint32_t s2, uint32_t w, uint32_t h)
 {
@@ -180,6 +208,8 @@ bool plane_fits(const svgf_plane& pl, uint32_t h) { return pl.pitch_bytes * (uin
 
 void free_slot(BatchSlot& s)
 {
+    for (auto& e : s.sev)
+        if (e) (void)hipEventDestroy(e);
     if (s.exec) (void)hipGraphExecDestroy(s.exec);
     if (s.graph) (void)hipGraphDestroy(s.graph);
     if (s.done) (void)hipEventDestroy(s.done);
@@ -205,6 +235,50 @@ void drop_graphs(svgf_context* c)
     }
 }
 
+// Reads the outcome of a slot's batch once it has completed (the caller has seen its event):
+// when a dependency wait of its persistent launch gave up, its outputs and the state it wrote
+// are wrong, so its streams restart with cold history and never reproject that state.
+void check_slot(svgf_context* c, BatchSlot& s)
+{
+    if (s.pending || s.checked || !s.seq) return;
+    s.checked = true;
+    if (s.sev_n > 0) {
+        // stage times of a batch recorded with SVGF_BATCH_TIMED_STAGES
+        float t[MAX_STAGES];
+        bool ok = true;
+        for (int i = 0; i < s.sev_n && ok; ++i) ok = hipEventElapsedTime(&t[i], s.sev[i], s.sev[i + 1]) == hipSuccess;
+        if (ok) {
+            if (c->stage_n != s.sev_n) {
+                std::fill(std::begin(c->stage_sum), std::end(c->stage_sum), 0.0);
+                c->stage_batches = 0;
+                c->stage_n = s.sev_n;
+            }
+            for (int i = 0; i < s.sev_n; ++i) c->stage_sum[i] += t[i];
+            c->stage_batches++;
+        }
+    }
+    if (!s.err_host || *s.err_host == 0) return;
+    *s.err_host = 0;
+    s.failed = true;
+    for (uint32_t id : s.ids)
+        if (id < c->streams.size() && c->streams[id].open) c->streams[id].frames = 0;
+}
+
+// Marks completed slots (non-blocking) and reads their outcome.
+svgf_status poll_slots(svgf_context* c)
+{
+    for (auto& s : c->slots) {
+        if (s.pending) {
+            const hipError_t q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) return SVGF_ERR_HIP;
+            s.pending = false;
+        }
+        check_slot(c, s);
+    }
+    return SVGF_OK;
+}
+
 void destroy_context(svgf_context* c)
 {
     (void)hipDeviceSynchronize();
@@ -220,6 +294,13 @@ void destroy_context(svgf_context* c)
 }
 
 int stage_count(const svgf_context* c) { return c->kp.iterations + 1; }
+
+// Resident workgroups of the persistent kernel variant the parameters select.
+void size_persistent(svgf_context* c)
+{
+    const int per_cu = persistent_blocks_per_cu(c->mode == SVGF_MODE_CANONICAL, c->st, state_form(c->kp));
+    c->persist_blocks = (uint32_t)std::max(1, per_cu * c->cus);
+}
 
 // One persistent launch per batch (stage profiling needs one launch per stage).
 bool persistent(const svgf_context* c)
@@ -252,17 +333,82 @@ hipEvent_t* profile_events(svgf_context* c)
     return c->ev;
 }
 
-hipError_t enqueue_batch(svgf_context* c, BatchSlot& s, const uint32_t* tiles, hipStream_t stream,
-                         hipEvent_t* ev = nullptr)
+bool timed_stages(const svgf_context* c)
+{
+    return (c->desc.batch_flags & SVGF_BATCH_TIMED_STAGES) && !(c->desc.batch_flags & SVGF_PROFILE_STAGES);
+}
+
+// Events around the stage launches of a batch in slot s launched directly: the context's
+// (SVGF_PROFILE_STAGES) or the slot's own (SVGF_BATCH_TIMED_STAGES). A captured batch gets the
+// slot's events as graph nodes instead (add_stage_events): stream capture does not turn
+// hipEventRecord into a graph node.
+hipEvent_t* batch_events(svgf_context* c, BatchSlot& s, bool capturing)
+{
+    if (hipEvent_t* ev = profile_events(c)) return ev;
+    if (!timed_stages(c) || capturing) return nullptr;
+    s.sev_n = launches_per_batch(c);
+    return s.sev;
+}
+
+// Inserts an event-record node before every kernel node of a captured chain (table copy ->
+// kernel 0 -> kernel 1 -> ...) and one after the last: ev[i] is recorded before kernel i and
+// ev[n] after kernel n - 1, as direct launches record them. Returns the number of kernels.
+hipError_t add_stage_events(hipGraph_t g, hipEvent_t* ev, int max_kernels, int& kernels)
+{
+    std::vector<hipGraphNode_t> chain;
+    hipGraphNode_t node = nullptr;
+    size_t n = 1;
+    hipError_t e = hipGraphGetRootNodes(g, &node, &n);
+    while (e == hipSuccess && n == 1 && node) {
+        chain.push_back(node);
+        size_t nd = 0;
+        e = hipGraphNodeGetDependentNodes(node, nullptr, &nd);
+        if (e != hipSuccess || nd == 0) break;
+        if (nd != 1) return hipErrorUnknown;
+        hipGraphNode_t next = nullptr;
+        e = hipGraphNodeGetDependentNodes(node, &next, &nd);
+        node = next;
+    }
+    kernels = 0;
+    hipGraphNode_t last = nullptr;
+    for (size_t i = 0; i < chain.size() && e == hipSuccess; ++i) {
+        hipGraphNodeType t;
+        if ((e = hipGraphNodeGetType(chain[i], &t)) != hipSuccess) break;
+        if (t != hipGraphNodeTypeKernel) continue;
+        if (kernels >= max_kernels) return hipErrorUnknown;
+        hipGraphNode_t en = nullptr;
+        if (i > 0) {
+            if ((e = hipGraphRemoveDependencies(g, &chain[i - 1], &chain[i], 1)) != hipSuccess) break;
+            e = hipGraphAddEventRecordNode(&en, g, &chain[i - 1], 1, ev[kernels]);
+        } else {
+            e = hipGraphAddEventRecordNode(&en, g, nullptr, 0, ev[kernels]);
+        }
+        if (e == hipSuccess) e = hipGraphAddDependencies(g, &en, &chain[i], 1);
+        ++kernels;
+        last = chain[i];
+    }
+    if (e == hipSuccess && last) {
+        hipGraphNode_t en = nullptr;
+        e = hipGraphAddEventRecordNode(&en, g, &last, 1, ev[kernels]);
+    }
+    return e;
+}
+
+hipError_t enqueue_batch(svgf_context* c, BatchSlot& s, const uint32_t* tiles, hipStream_t stream, bool capturing = false)
 {
     hipError_t e = hipMemcpyAsync(s.table, s.staging, c->table_bytes, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return e;
     char* t = static_cast<char*>(s.table);
     BatchTable bt{reinterpret_cast<const uint32_t*>(t), reinterpret_cast<const uint32_t*>(t + c->starts_off),
                   reinterpret_cast<const FrameDesc*>(t + c->frames_off)};
-    if (persistent(c))
-        return launch_batch_persistent(c->mode == SVGF_MODE_CANONICAL, bt, c->kp, c->st, s.ctl, s.err_dev, c->max_polls,
-                                       launch_grid(c, tiles, 0), stream);
+    hipEvent_t* ev = batch_events(c, s, capturing);
+    if (persistent(c)) {
+        if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
+        e = launch_batch_persistent(c->mode == SVGF_MODE_CANONICAL, bt, c->kp, c->st, s.ctl, s.err_dev, c->max_polls,
+                                    launch_grid(c, tiles, 0), stream);
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
+        return e;
+    }
     const uint32_t* grids = tiles;
     if (c->mode == SVGF_MODE_REFERENCE) return launch_reference_batch(bt, c->kp, c->st, grids, stream, ev);
     return launch_canonical_batch(bt, c->kp, c->st, grids, stream, ev);
@@ -274,9 +420,15 @@ hipError_t capture_slot(svgf_context* c, BatchSlot& s, const uint32_t* tiles)
 {
     hipError_t e = hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    hipError_t le = enqueue_batch(c, s, tiles, c->capture_stream);
+    hipError_t le = enqueue_batch(c, s, tiles, c->capture_stream, true);
     e = hipStreamEndCapture(c->capture_stream, &s.graph);
     if (le != hipSuccess) e = le;
+    if (e == hipSuccess && timed_stages(c)) {
+        int kernels = 0;
+        e = add_stage_events(s.graph, s.sev, MAX_STAGES, kernels);
+        if (e == hipSuccess && kernels != launches_per_batch(c)) e = hipErrorUnknown;
+        s.sev_n = e == hipSuccess ? kernels : 0;
+    }
     if (e == hipSuccess) e = hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0);
     // kernel nodes in launch order: follow the dependency chain from the root (the copy)
     s.knodes.clear();
@@ -365,7 +517,9 @@ svgf_status svgf_default_params(uint32_t mode, uint32_t storage, svgf_params* ou
         p.depth_reject = 0.1f;
         p.normal_reject = 0.9f;
         p.history_cap = 32;
-        p.flags = storage == SVGF_STORAGE_FP16 ? SVGF_FLAG_COMPENSATED_MOMENTS : 0u;
+        // FP16: compensated moments, and the wide state so that binary16 roundings of the
+        // fed-back history and moments do not compound over frames
+        p.flags = storage == SVGF_STORAGE_FP16 ? SVGF_FLAG_COMPENSATED_MOMENTS | SVGF_FLAG_WIDE_STATE : 0u;
     }
     *out = p;
     return SVGF_OK;
@@ -374,7 +528,7 @@ svgf_status svgf_default_params(uint32_t mode, uint32_t storage, svgf_params* ou
 svgf_status svgf_query_bytes(const svgf_desc* d, uint32_t w, uint32_t h, uint64_t* context_bytes, uint64_t* stream_bytes)
 {
     if (!d || d->mode > SVGF_MODE_CANONICAL || d->storage > SVGF_STORAGE_FP16) return SVGF_ERR_INVALID_ARGUMENT;
-    uint32_t s4 = d->storage == SVGF_STORAGE_FP32 ? 16 : 8, s2 = d->storage == SVGF_STORAGE_FP32 ? 8 : 4;
+    uint32_t s4 = d->storage == SVGF_STORAGE_FP32 ? 16 : 8, s2 = moment_bytes(d->storage, d->params.flags);
     const uint32_t w4 = d->mode == SVGF_MODE_CANONICAL ? 16 : s4;
     if (context_bytes) {
         uint64_t b = frame_scratch_bytes(w4, (uint64_t)d->max_width * d->max_height);
@@ -401,8 +555,9 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
     if ((uint64_t)d->max_width * d->max_height * 16 >= (1ull << 31)) return SVGF_ERR_CAPACITY;
     if (d->max_batch_frames > MAX_BATCH_FRAMES) return SVGF_ERR_INVALID_ARGUMENT;
     if (d->max_batch_frames && d->max_batch_pixels < 1) return SVGF_ERR_INVALID_ARGUMENT;
-    if (d->batch_flags & ~(SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES | SVGF_BATCH_PERSISTENT)) return SVGF_ERR_INVALID_ARGUMENT;
-    svgf_status ps = validate_params(d->params, d->mode);
+    if (d->batch_flags & ~(SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES | SVGF_BATCH_PERSISTENT | SVGF_BATCH_TIMED_STAGES))
+        return SVGF_ERR_INVALID_ARGUMENT;
+    svgf_status ps = validate_params(d->params, d->mode, d->storage);
     if (ps != SVGF_OK) return ps;
 
     int ndev = 0;
@@ -418,13 +573,13 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
     c->mode = (int)d->mode;
     c->st = d->storage == SVGF_STORAGE_FP32 ? Storage::F32 : Storage::F16;
     c->s4 = c->st == Storage::F32 ? 16 : 8;
-    c->s2 = c->st == Storage::F32 ? 8 : 4;
+    c->s2 = moment_bytes(d->storage, d->params.flags);
     c->w4 = c->mode == SVGF_MODE_CANONICAL ? 16 : c->s4;
     c->kp = to_kparams(d->params);
     c->grid_blocks = std::max(1, prop.multiProcessorCount) * 8;
+    c->cus = std::max(1, prop.multiProcessorCount);
     {
-        const int per_cu = persistent_blocks_per_cu(c->mode == SVGF_MODE_CANONICAL, c->st, c->kp.compensated != 0);
-        c->persist_blocks = (uint32_t)std::max(1, per_cu * std::max(1, prop.multiProcessorCount));
+        size_persistent(c);
         // dependency-wait bound of the persistent kernel, in polls of about 1.3 us (a 1024-cycle
         // sleep plus one load): 250 ms
         uint64_t us = 250000;
@@ -473,6 +628,13 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
                 return SVGF_ERR_OUT_OF_MEMORY;
             }
             *s.err_host = 0;
+            if (d->batch_flags & SVGF_BATCH_TIMED_STAGES)
+                for (auto& e : s.sev)
+                    if (hipEventCreate(&e) != hipSuccess) {
+                        e = nullptr;
+                        destroy_context(c);
+                        return SVGF_ERR_HIP;
+                    }
         }
         if (hipDeviceSynchronize() != hipSuccess) {
             destroy_context(c);
@@ -498,12 +660,19 @@ svgf_status svgf_destroy(svgf_context* c)
 svgf_status svgf_set_params(svgf_context* c, const svgf_params* p)
 {
     if (!c || !p) return SVGF_ERR_INVALID_ARGUMENT;
-    svgf_status s = validate_params(*p, c->desc.mode);
+    svgf_status s = validate_params(*p, c->desc.mode, c->desc.storage);
     if (s != SVGF_OK) return s;
+    // the flags select how resident streams store their moments and history: open streams
+    // would read their state in the other representation
+    if (p->flags != c->desc.params.flags &&
+        std::any_of(c->streams.begin(), c->streams.end(), [](const StreamSlot& t) { return t.open; }))
+        return SVGF_ERR_INVALID_ARGUMENT;
     DeviceGuard guard(c->desc.device);
     drop_graphs(c);  // captured kernels hold the old parameters by value
     c->desc.params = *p;
     c->kp = to_kparams(*p);
+    c->s2 = moment_bytes(c->desc.storage, p->flags);
+    size_persistent(c);
     return SVGF_OK;
 }
 
@@ -630,6 +799,7 @@ svgf_status svgf_stream_close(svgf_context* c, uint32_t id)
         if (sl.pending && std::find(sl.ids.begin(), sl.ids.end(), id) != sl.ids.end()) {
             (void)hipEventSynchronize(sl.done);
             sl.pending = false;
+            check_slot(c, sl);  // before the id can name another stream
         }
     c->pool.emplace(s->bytes, s->block);
     c->pool_bytes += s->bytes;
@@ -702,16 +872,26 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
     if (total_px > c->desc.max_batch_pixels || arena_need > c->arena_bytes) return SVGF_ERR_CAPACITY;
 
     DeviceGuard guard(c->desc.device);
+    if (persistent(c)) {
+        // A batch that reads the state a batch in flight writes waits for it, so that a failed
+        // launch (its streams then restart cold) is found before its state is read.
+        for (auto& sl : c->slots) {
+            if (!sl.pending) continue;
+            bool reads = false;
+            for (uint32_t i = 0; i < n && !reads; ++i)
+                reads = std::find(sl.ids.begin(), sl.ids.end(), req[i].stream_id) != sl.ids.end();
+            if (!reads) continue;
+            if (hipEventSynchronize(sl.done) != hipSuccess) return SVGF_ERR_HIP;
+            sl.pending = false;
+        }
+    }
+    if (poll_slots(c) != SVGF_OK) return SVGF_ERR_HIP;
     BatchSlot& slot = c->slots[c->next_slot];
     c->next_slot = (c->next_slot + 1) % BATCH_SLOTS;
     if (slot.pending) {
         if (hipEventSynchronize(slot.done) != hipSuccess) return SVGF_ERR_HIP;
         slot.pending = false;
-    }
-    if (*slot.err_host) {
-        // a dependency wait of this slot's last persistent launch gave up
-        *slot.err_host = 0;
-        return SVGF_ERR_TIMEOUT;
+        check_slot(c, slot);
     }
     char* host = static_cast<char*>(slot.staging);
     uint32_t* hdr = reinterpret_cast<uint32_t*>(host);
@@ -755,7 +935,7 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
     hipStream_t stream = static_cast<hipStream_t>(hip_stream);
     hipError_t e = hipSuccess;
     if (c->desc.batch_flags & (SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES)) {
-        e = enqueue_batch(c, slot, tiles, stream, profile_events(c));
+        e = enqueue_batch(c, slot, tiles, stream);
     } else {
         if (!slot.exec) {
             if (capture_slot(c, slot, tiles) != hipSuccess) return SVGF_ERR_HIP;
@@ -772,6 +952,9 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
     if (e != hipSuccess) return SVGF_ERR_HIP;
     if (hipEventRecord(slot.done, stream) != hipSuccess) return SVGF_ERR_HIP;
     slot.pending = true;
+    slot.seq = ++c->last_seq;
+    slot.checked = false;
+    slot.failed = false;
     slot.ids.clear();
     for (uint32_t i = 0; i < n; ++i) slot.ids.push_back(req[i].stream_id);
     for (uint32_t i = 0; i < n; ++i) {
@@ -782,11 +965,53 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
     return SVGF_OK;
 }
 
+svgf_status svgf_batch_last(svgf_context* c, uint64_t* batch)
+{
+    if (!c || !batch) return SVGF_ERR_INVALID_ARGUMENT;
+    *batch = c->last_seq;
+    return SVGF_OK;
+}
+
+svgf_status svgf_batch_status(svgf_context* c, uint64_t batch, int32_t wait)
+{
+    if (!c || batch == 0) return SVGF_ERR_INVALID_ARGUMENT;
+    BatchSlot* s = nullptr;
+    for (auto& sl : c->slots)
+        if (sl.seq == batch) s = &sl;
+    if (!s) return SVGF_ERR_INVALID_ARGUMENT;  // never enqueued, or its slot was reused since
+    DeviceGuard guard(c->desc.device);
+    if (s->pending) {
+        const hipError_t q = wait ? hipEventSynchronize(s->done) : hipEventQuery(s->done);
+        if (q == hipErrorNotReady) return SVGF_ERR_NOT_READY;
+        if (q != hipSuccess) return SVGF_ERR_HIP;
+        s->pending = false;
+    }
+    check_slot(c, *s);
+    return s->failed ? SVGF_ERR_TIMEOUT : SVGF_OK;
+}
+
 svgf_status svgf_stage_times(svgf_context* c, float* ms, uint32_t capacity, uint32_t* count)
 {
     if (!c || !count || (!ms && capacity)) return SVGF_ERR_INVALID_ARGUMENT;
-    if (!(c->desc.batch_flags & SVGF_PROFILE_STAGES) || c->ev_stages == 0) return SVGF_ERR_UNSUPPORTED;
     DeviceGuard guard(c->desc.device);
+    if (!(c->desc.batch_flags & SVGF_PROFILE_STAGES) && (c->desc.batch_flags & SVGF_BATCH_TIMED_STAGES)) {
+        // mean over the batches completed since the previous call (waits for those in flight)
+        for (auto& s : c->slots) {
+            if (s.pending) {
+                if (hipEventSynchronize(s.done) != hipSuccess) return SVGF_ERR_HIP;
+                s.pending = false;
+            }
+            check_slot(c, s);
+        }
+        if (c->stage_batches == 0) return SVGF_ERR_NOT_READY;
+        *count = (uint32_t)c->stage_n;
+        for (int i = 0; i < c->stage_n && (uint32_t)i < capacity; ++i)
+            ms[i] = (float)(c->stage_sum[i] / (double)c->stage_batches);
+        std::fill(std::begin(c->stage_sum), std::end(c->stage_sum), 0.0);
+        c->stage_batches = 0;
+        return SVGF_OK;
+    }
+    if (!(c->desc.batch_flags & SVGF_PROFILE_STAGES) || c->ev_stages == 0) return SVGF_ERR_UNSUPPORTED;
     *count = (uint32_t)c->ev_stages;
     if (hipEventSynchronize(c->ev[c->ev_stages]) != hipSuccess) return SVGF_ERR_HIP;
     for (int i = 0; i < c->ev_stages && (uint32_t)i < capacity; ++i)

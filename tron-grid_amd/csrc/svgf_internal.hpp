@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "svgf_sched.hpp"
+#include "svgf_state.hpp"
 
 namespace svgf {
 
@@ -47,6 +48,7 @@ struct KParams {
     int32_t history_cap;
     int32_t iterations;
     int32_t compensated;
+    int32_t wide;  // FP16 CANONICAL wide state (svgf_state.hpp)
     // Canonical a-trous edge-stop constants, folded on the host so the exponent of a tap is
     // two fused multiply-adds (the filter works in base 2):
     //   luma_k  = -log2(e) / phi_luma
@@ -66,6 +68,11 @@ struct BatchTable {
 };
 
 enum class Storage : int { F32 = 0, F16 = 1 };
+
+// Form of the CANONICAL temporal state: raw moments (mu1, mu2), compensated (mean, sigma), or
+// compensated with the wide FP16 history and moment words of svgf_state.hpp.
+enum StateForm : int { SF_RAW = 0, SF_COMP = 1, SF_WIDE = 2 };
+inline int state_form(const KParams& p) { return p.wide ? SF_WIDE : (p.compensated ? SF_COMP : SF_RAW); }
 
 // Stage launchers (svgf_kernels.hip). All enqueue on `stream` and return hipGetLastError().
 // `ev` (optional) receives stages + 1 events: ev[0] before the first stage, ev[i + 1] after
@@ -88,7 +95,7 @@ hipError_t launch_canonical_batch(const BatchTable& t, const KParams& p, Storage
 hipError_t launch_batch_persistent(int canonical, const BatchTable& t, const KParams& p, Storage st, PersistCtl* ctl,
                                    uint32_t* err, uint64_t max_polls, uint32_t blocks, hipStream_t stream);
 // Resident workgroups per CU of the persistent kernel (0 if the query failed).
-int persistent_blocks_per_cu(int canonical, Storage st, bool compensated);
+int persistent_blocks_per_cu(int canonical, Storage st, int state_form);
 
 // Makes `dev` current for the scope of the object and restores the caller's device after.
 struct DeviceGuard {

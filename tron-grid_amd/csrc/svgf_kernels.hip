@@ -135,6 +135,17 @@ struct F16S {
         v0 = make_float2((float)h0.x, (float)h0.y);
         v1 = make_float2((float)h1.x, (float)h1.y);
     }
+    // raw 64-bit words of pixels x and x + 1 (r1 = r0 when `first`)
+    __device__ static __forceinline__ void ld_raw4x2(const Buf& b, int x, int y, bool first, u32x2_t& r0, u32x2_t& r1)
+    {
+        const u32x4_t r = __builtin_amdgcn_raw_buffer_load_b128(b.r, y * b.pitch + x * B4, 0, 0);
+        r0 = u32x2_t{r.x, r.y};
+        r1 = first ? r0 : u32x2_t{r.z, r.w};
+    }
+    __device__ static __forceinline__ void st_raw4(const Buf& b, int x, int y, half4_t h)
+    {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, h), b.r, y * b.pitch + x * B4, 0, 0);
+    }
     __device__ static __forceinline__ float lda(const Buf& b, int x, int y)
     {
         return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(b.r, y * b.pitch + x * B4 + 6, 0, 0));
@@ -153,6 +164,38 @@ __device__ __forceinline__ void stg4(const Buf& b, int x, int y, float4 v) { st_
 __device__ __forceinline__ float2 ldg2(const Buf& b, int x, int y) { return ld_f2(b, y * b.pitch + x * 8); }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Wide state of the FP16 CANONICAL form (word layout in svgf_state.hpp).
+// History pixel from its raw 64-bit word: colour decoded with its residual codes, length n.
+__device__ __forceinline__ float4 wide_history(u32x2_t r, uint32_t& n)
+{
+    const half4_t h = __builtin_bit_cast(half4_t, r);
+    const uint16_t word = (uint16_t)(r.y >> 16);
+    n = wide_history_length(word);
+    return make_float4(wide_decode_channel(h.x, wide_history_code(word, 0)),
+                       wide_decode_channel(h.y, wide_history_code(word, 1)),
+                       wide_decode_channel(h.z, wide_history_code(word, 2)), 0.0f);
+}
+__device__ __forceinline__ void st_wide_history(const Buf& b, int x, int y, float4 o, uint32_t n)
+{
+    uint32_t q0, q1, q2;
+    const _Float16 h0 = wide_encode_channel(o.x, q0), h1 = wide_encode_channel(o.y, q1), h2 = wide_encode_channel(o.z, q2);
+    const half4_t h = {h0, h1, h2, __builtin_bit_cast(_Float16, wide_history_word(n, q0, q1, q2))};
+    F16S::st_raw4(b, x, y, h);
+}
+// Moments pixel (mean, sigma) from its raw (hi, hi, lo, lo) binary16 words.
+__device__ __forceinline__ float2 wide_moments(u32x2_t r)
+{
+    const half4_t h = __builtin_bit_cast(half4_t, r);
+    return make_float2(wide_decode_moment(h.x, h.z), wide_decode_moment(h.y, h.w));
+}
+__device__ __forceinline__ void st_wide_moments(const Buf& b, int x, int y, float mu, float sd)
+{
+    _Float16 mh, ml, sh, sl;
+    wide_encode_moment(mu, mh, ml);
+    wide_encode_moment(sd, sh, sl);
+    F16S::st_raw4(b, x, y, half4_t{mh, sh, ml, sl});
+}
 
 // Completes this wave's outstanding LDS operations (s_waitcnt lgkmcnt(0)). A value one thread
 // writes to LDS for the other waves must have landed before that thread's wave reaches the
@@ -380,10 +423,12 @@ __device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(
 // also keeps G for the next frame. Standard deviations rather than variances are stored so
 // that heavy-tailed radiance stays inside the FP16 range.
 // Pixel (x, y) of the frame, with its G and the depths of its four clamped neighbours.
-template <class S, bool COMP>
+template <class S, int SF>
 __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KParams& p, int x, int y, float4 G,
                                                    float zl, float zr, float zu, float zd)
 {
+    constexpr bool COMP = SF != SF_RAW, WIDE = SF == SF_WIDE;
+    static_assert(!WIDE || S::B4 == 8, "the wide state is a binary16 form");
     float4 R = S::ld4(bind(f.rad, f.h), x, y);
     float l = luma_fma(R.x, R.y, R.z);
     if (f.gsave.ptr) stg4(bind(f.gsave, f.h), x, y, G);
@@ -410,8 +455,11 @@ __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KPa
         // dependent memory round trip instead of three. History and moments come as the pixel
         // pair (px, px + 1) of each row, px = clamp(x0): when x0 = -1 the right tap (x = 0) is
         // the pair's first pixel; every other in-frame tap is the pixel it names.
+        // (wide form: the history words carry the residual codes and the length; the moments
+        // are hi / lo pairs, two RGBA16F pixels per load)
         float4 Gq[4], H[4];
         float2 M[4];
+        uint32_t NQ[4];
         const int px = clampi(x0, 0, f.w - 1);
         const bool right_first = x0 < 0;
 #pragma unroll
@@ -419,8 +467,20 @@ __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KPa
             const int cy = clampi(y0 + jj, 0, f.h - 1);
 #pragma unroll
             for (int ii = 0; ii < 2; ++ii) Gq[jj * 2 + ii] = ldg4(gp, clampi(x0 + ii, 0, f.w - 1), cy);
-            S::ld4x2(hp, px, cy, right_first, H[jj * 2], H[jj * 2 + 1]);
-            S::ld2x2(mp, px, cy, right_first, M[jj * 2], M[jj * 2 + 1]);
+            if constexpr (WIDE) {
+                u32x2_t r0, r1;
+                F16S::ld_raw4x2(hp, px, cy, right_first, r0, r1);
+                H[jj * 2] = wide_history(r0, NQ[jj * 2]);
+                H[jj * 2 + 1] = wide_history(r1, NQ[jj * 2 + 1]);
+                F16S::ld_raw4x2(mp, px, cy, right_first, r0, r1);
+                M[jj * 2] = wide_moments(r0);
+                M[jj * 2 + 1] = wide_moments(r1);
+            } else {
+                S::ld4x2(hp, px, cy, right_first, H[jj * 2], H[jj * 2 + 1]);
+                S::ld2x2(mp, px, cy, right_first, M[jj * 2], M[jj * 2 + 1]);
+                NQ[jj * 2] = (uint32_t)(int)H[jj * 2].w;
+                NQ[jj * 2 + 1] = (uint32_t)(int)H[jj * 2 + 1].w;
+            }
         }
 #pragma unroll
         for (int jj = 0; jj < 2; ++jj) {
@@ -433,7 +493,7 @@ __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KPa
                 float dz = fabsf(Gq[t].w - G.w);
                 if (__builtin_fmaf(-p.depth_reject, G.w, dz) > 0.0f) continue;
                 if (!dot_at_least(Gq[t], G, p.normal_reject)) continue;
-                int nq = (int)H[t].w;
+                const int nq = (int)NQ[t];
                 if (nq < 1) continue;
                 sumb += b;
                 h0 = fmaf(b, H[t].x, h0);
@@ -489,7 +549,8 @@ __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KPa
         float d = l - hmu;
         float mu = fmaf(am, d, hmu);
         float m2 = (1.0f - am) * fmaf(am * d, d, hvar);
-        S::st2(mc, x, y, make_float2(mu, sqrt_hw(m2)));
+        if constexpr (WIDE) st_wide_moments(mc, x, y, mu, sqrt_hw(m2));
+        else S::st2(mc, x, y, make_float2(mu, sqrt_hw(m2)));
         var = m2;
     } else {
         float mu1 = fmaf(am, l - hm1, hm1);
@@ -502,7 +563,7 @@ __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KPa
     st_u8(nl, y * nl.pitch + x, (uint32_t)n_new);
 }
 
-template <class S, bool COMP, class L>
+template <class S, int SF, class L>
 __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KParams& p, const Tile& T)
 {
     const int tid = lane_tid();
@@ -535,7 +596,7 @@ __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KPar
     __syncthreads();
     if (x <= max_x && y <= max_y) {
         const int c = (tj + 1) * ZW + ti + 1;
-        can_temporal_pixel<S, COMP>(f, p, x, y, G, sZ[c - 1], sZ[c + 1], sZ[c - ZW], sZ[c + ZW]);
+        can_temporal_pixel<S, SF>(f, p, x, y, G, sZ[c - 1], sZ[c + 1], sZ[c - ZW], sZ[c + ZW]);
     }
     __syncthreads();  // the next tile of a grid-stride loop reuses sZ
 }
@@ -644,7 +705,9 @@ __device__ __forceinline__ float tap_weight(const KParams& p, const Centre& c, i
     return __builtin_amdgcn_exp2f(e);
 }
 
-// Weighted sums of a pixel: colour (3), weight, squared-weight variance.
+// Weighted sums of a pixel: colour (3), weight and, unless VAR is false (the last iteration,
+// whose output carries no variance), the squared-weight variance.
+template <bool VAR>
 struct Sums {
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sw = 0.0f, sv = 0.0f;
     __device__ __forceinline__ void add(float w, float4 A, float V)
@@ -653,20 +716,25 @@ struct Sums {
         s1 = fmaf(w, A.y, s1);
         s2 = fmaf(w, A.z, s2);
         sw += w;
-        sv = fmaf(w * w, V, sv);
+        if constexpr (VAR) sv = fmaf(w * w, V, sv);
     }
     // The row's sums are complete here. Without this the optimiser sinks the weighted sums
     // below later rows' exponents and keeps those taps' weights and colours live (77 VGPRs
     // instead of 58 in the one-pixel filter, 6 waves per SIMD instead of 8, a quarter slower).
-    __device__ __forceinline__ void pin() { asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(sw), "+v"(sv)); }
-    // (colour, sigma) of the filtered pixel; the centre's own when the weights vanish
+    __device__ __forceinline__ void pin()
+    {
+        if constexpr (VAR) asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(sw), "+v"(sv));
+        else asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2), "+v"(sw));
+    }
+    // (colour, sigma) of the filtered pixel; the centre's own when the weights vanish (sigma 0
+    // without VAR)
     __device__ __forceinline__ float4 result(float4 C, float Vc) const
     {
         if (sw > 1e-6f) {
             const float inv = rcp_hw(sw);
-            return make_float4(s0 * inv, s1 * inv, s2 * inv, sqrt_hw(sv) * inv);
+            return make_float4(s0 * inv, s1 * inv, s2 * inv, VAR ? sqrt_hw(sv) * inv : 0.0f);
         }
-        return make_float4(C.x, C.y, C.z, sqrt_hw(Vc));
+        return make_float4(C.x, C.y, C.z, VAR ? sqrt_hw(Vc) : 0.0f);
     }
 };
 
@@ -685,10 +753,14 @@ struct Sums {
 // TALL and NR = 2 (blocked): a thread filters two vertically adjacent lattice points, whose 5x5
 // taps share 4 of their 5 rows: 6 rows of LDS entries serve both. Otherwise (the 16x16 tiles
 // of single small frames) a thread filters one lattice point.
-template <class S, bool FIRST, int NR, bool TALL, class L>
+// LAST: the caller's k is iterations - 1 >= 1; the output carries no variance, so the taps
+// skip the squared-weight sum and its LDS reads. SF: the moment / history form (the wide form
+// encodes the colour history written by FIRST).
+template <class S, int SF, bool FIRST, bool LAST, int NR, bool TALL, class L>
 __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
 {
     static_assert(!FIRST || NR == 1, "the first iteration runs at stride 1");
+    static_assert(!(FIRST && LAST), "the first iteration keeps its runtime last-iteration test");
     static_assert(!TALL || FIRST, "tall tiles are the first iteration's");
     constexpr bool BLK = TALL || NR == 2;
     constexpr int H = FIRST ? LHALO + 1 : LHALO;          // staged halo
@@ -805,8 +877,11 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     const Buf nl = bind(f.nlen, f.h);
     auto length = [&](int x, int y) { return FIRST ? ld_u8(nl, y * nl.pitch + x) : 0u; };
     auto store = [&](int x, int y, float4 o, uint32_t n) {
-        if constexpr (FIRST) S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, (float)n));
-        if (k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
+        if constexpr (FIRST) {
+            if constexpr (SF == SF_WIDE) st_wide_history(bind(f.hist_curr, f.h), x, y, o, n);
+            else S::st4(bind(f.hist_curr, f.h), x, y, make_float4(o.x, o.y, o.z, (float)n));
+        }
+        if (LAST || k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
         else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
     };
     if constexpr (!BLK) {
@@ -819,14 +894,14 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
                                    : ld_f2(gr, y * gr.pitch + x * 8);
             const uint32_t n = length(x, y);
             const Centre c = centre_of(p, C, sB[c0], prefilter(c0), g.x, g.y, s);
-            Sums a;
+            Sums<!LAST> a;
 #pragma unroll
             for (int dy = -2; dy <= 2; ++dy) {
 #pragma unroll
                 for (int dx = -2; dx <= 2; ++dx) {
                     const int q = c0 + dy * RW + dx;
                     const float4 A = sA[q];
-                    a.add(tap_weight(p, c, dx, dy, A, sB[q]), A, sV[q]);
+                    a.add(tap_weight(p, c, dx, dy, A, sB[q]), A, LAST ? 0.0f : sV[q]);
                 }
                 a.pin();
             }
@@ -852,14 +927,14 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             const uint32_t n0 = length(x, y0), n1 = length(x, min(y1, max_y));
             const Centre e0 = centre_of(p, C0, sB[c0], prefilter(c0), g0.x, g0.y, s);
             const Centre e1 = centre_of(p, C1, sB[c1], prefilter(c1), g1.x, g1.y, s);
-            Sums a0, a1;
+            Sums<!LAST> a0, a1;
 #pragma unroll
             for (int j = -2; j <= 3; ++j) {
 #pragma unroll
                 for (int dx = -2; dx <= 2; ++dx) {
                     const int q = c0 + j * RW + dx;
                     const float4 A = sA[q], B = sB[q];
-                    const float V = sV[q];
+                    const float V = LAST ? 0.0f : sV[q];
                     if (j <= 2) a0.add(tap_weight(p, e0, dx, j, A, B), A, V);
                     if (j >= -1) a1.add(tap_weight(p, e1, dx, j - 1, A, B), A, V);
                 }
@@ -878,19 +953,24 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
 // ========================================================================================
 enum StageId : int {
     ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_ATROUS0 = 3, ST_CAN_ATROUS = 4, ST_CAN_ATROUS_PAIR = 5,
-    ST_CAN_ATROUS0_SQ = 6  // the first a-trous iteration on 16x16 tiles (single small frames)
+    ST_CAN_ATROUS0_SQ = 6,  // the first a-trous iteration on 16x16 tiles (single small frames)
+    // the last iteration (k = iterations - 1 >= 1): writes the output, no filtered variance
+    ST_CAN_ATROUS_L = 7, ST_CAN_ATROUS_PAIR_L = 8
 };
+constexpr bool is_pair_stage(int st) { return st == ST_CAN_ATROUS_PAIR || st == ST_CAN_ATROUS_PAIR_L; }
 
-template <class S, int STAGE, bool COMP, class L = OwnLds>
+template <class S, int STAGE, int SF, class L = OwnLds>
 __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, const Tile& T, int arg)
 {
     if constexpr (STAGE == ST_REF_INIT) ref_init_tile<S>(f, p, T);
     else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_tile<S, L>(f, p, T, arg);
-    else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, COMP, L>(f, p, T);
-    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, true, 1, true, L>(f, p, T, 0);
-    else if constexpr (STAGE == ST_CAN_ATROUS0_SQ) can_atrous_tile<S, true, 1, false, L>(f, p, T, 0);
-    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, false, 1, false, L>(f, p, T, arg);
-    else can_atrous_tile<S, false, 2, false, L>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, SF, L>(f, p, T);
+    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, SF, true, false, 1, true, L>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS0_SQ) can_atrous_tile<S, SF, true, false, 1, false, L>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, SF, false, false, 1, false, L>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS_L) can_atrous_tile<S, SF, false, true, 1, false, L>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS_PAIR_L) can_atrous_tile<S, SF, false, true, 2, false, L>(f, p, T, arg);
+    else can_atrous_tile<S, SF, false, false, 2, false, L>(f, p, T, arg);
 }
 
 // Frame descriptor of a batch entry into scalar registers (the index is uniform across the block).
@@ -908,22 +988,22 @@ __device__ __forceinline__ FrameDesc load_frame(const FrameDesc* frames, uint32_
 template <int STAGE>
 __device__ __forceinline__ Tile stage_tile(int w, uint32_t log2s, uint32_t t)
 {
-    if constexpr (STAGE == ST_CAN_ATROUS_PAIR) return decode_pair_tile(w, log2s, t);
+    if constexpr (is_pair_stage(STAGE)) return decode_pair_tile(w, log2s, t);
     else if constexpr (STAGE == ST_CAN_ATROUS0) return decode_tall_tile(w, t);
     else return decode_tile(w, log2s, t);
 }
 
 // arg: REFERENCE a-trous pass (1..N) or CANONICAL a-trous iteration (0..N-1); log2s: stride.
-template <class S, int STAGE, bool COMP>
+template <class S, int STAGE, int SF>
 __global__ __launch_bounds__(BLOCK) void k_frame(FrameDesc f, KParams p, int arg, uint32_t log2s)
 {
     Tile T = stage_tile<STAGE>(f.w, log2s, xcd_logical(blockIdx.x, gridDim.x));
-    run_stage<S, STAGE, COMP>(f, p, T, arg);
+    run_stage<S, STAGE, SF>(f, p, T, arg);
 }
 
 // One block per tile of the batch (the host sizes the grid to the stage's tile count; a
 // grid-stride loop here would keep a second tile's state live and double the VGPR count).
-template <class S, int STAGE, bool COMP>
+template <class S, int STAGE, int SF>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchTable bt, KParams p, int arg, uint32_t log2s, int stage)
 {
     const uint32_t nf = bt.hdr[HDR_FRAMES];
@@ -933,19 +1013,19 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchTable bt, KParams p, int a
     const uint32_t fi = frame_of(starts, nf, t);
     const FrameDesc f = load_frame(bt.frames, fi);
     Tile T = stage_tile<STAGE>(f.w, log2s, t - starts[fi]);
-    run_stage<S, STAGE, COMP>(f, p, T, arg);
+    run_stage<S, STAGE, SF>(f, p, T, arg);
 }
 
 // One tile of a persistent launch. Each stage's branch loads the frame descriptor itself: one
 // load ahead of the branches would keep every descriptor field live across all of them and raise
 // the register count. The stage tiles share one LDS arena (ArenaLds).
-template <class S, int STAGE, bool COMP>
+template <class S, int STAGE, int SF>
 __device__ __forceinline__ void persist_tile(const FrameDesc* frames, uint32_t fi, const KParams& p, uint32_t log2s,
                                           uint32_t tile, int arg)
 {
     const FrameDesc f = load_frame(frames, fi);
     constexpr bool CANON = STAGE != ST_REF_INIT && STAGE != ST_REF_ATROUS;
-    run_stage<S, STAGE, COMP, ArenaLds<CANON>>(f, p, stage_tile<STAGE>(f.w, log2s, tile), arg);
+    run_stage<S, STAGE, SF, ArenaLds<CANON>>(f, p, stage_tile<STAGE>(f.w, log2s, tile), arg);
 }
 
 // Persistent form: every stage of a batch in one launch. Each workgroup takes chunks of
@@ -964,7 +1044,7 @@ __device__ __forceinline__ void persist_tile(const FrameDesc* frames, uint32_t f
 // sets *err (host memory) and ctl->abort, and skips its tile; once ctl->abort is set every
 // wait gives up at once, so the kernel always ends soon after. The last workgroup to find
 // the tickets exhausted clears the counters, leaving ctl zeroed for the next launch.
-template <class S, bool CANON, bool COMP>
+template <class S, bool CANON, int SF>
 __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, PersistCtl* ctl, uint32_t* err,
                                                    uint64_t max_polls)
 {
@@ -1039,14 +1119,21 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
         }
         if (run) {
             if constexpr (CANON) {
-                if (k.stage == 0) persist_tile<S, ST_CAN_TEMPORAL, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
-                else if (k.stage == 1) persist_tile<S, ST_CAN_ATROUS0, COMP>(bt.frames, k.frame, p, 0, k.tile, 0);
-                else if (PAIR_LOG2S > 1 && k.stage - 1 < PAIR_LOG2S)
-                    persist_tile<S, ST_CAN_ATROUS, COMP>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
-                else persist_tile<S, ST_CAN_ATROUS_PAIR, COMP>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
+                // stage i >= 2 is a-trous iteration i - 1 (residue pairs from stride 2^PAIR_LOG2S);
+                // the last one writes the output without the filtered variance
+                const bool single = PAIR_LOG2S > 1 && k.stage - 1 < PAIR_LOG2S, last = k.stage == nstages - 1;
+                if (k.stage == 0) persist_tile<S, ST_CAN_TEMPORAL, SF>(bt.frames, k.frame, p, 0, k.tile, 0);
+                else if (k.stage == 1) persist_tile<S, ST_CAN_ATROUS0, SF>(bt.frames, k.frame, p, 0, k.tile, 0);
+                else if (single && !last)
+                    persist_tile<S, ST_CAN_ATROUS, SF>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
+                else if (single)
+                    persist_tile<S, ST_CAN_ATROUS_L, SF>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
+                else if (!last)
+                    persist_tile<S, ST_CAN_ATROUS_PAIR, SF>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
+                else persist_tile<S, ST_CAN_ATROUS_PAIR_L, SF>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage - 1);
             } else {
-                if (k.stage == 0) persist_tile<S, ST_REF_INIT, false>(bt.frames, k.frame, p, 0, k.tile, 0);
-                else persist_tile<S, ST_REF_ATROUS, false>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage);
+                if (k.stage == 0) persist_tile<S, ST_REF_INIT, SF_RAW>(bt.frames, k.frame, p, 0, k.tile, 0);
+                else persist_tile<S, ST_REF_ATROUS, SF_RAW>(bt.frames, k.frame, p, k.stage - 1, k.tile, (int)k.stage);
             }
         }
         ++pending;
@@ -1074,21 +1161,21 @@ __global__ __launch_bounds__(BLOCK) void k_persist(BatchTable bt, KParams p, Per
         }
     }
 }
-template <class S, int STAGE, bool COMP>
+template <class S, int STAGE, int SF>
 void launch_frame(const FrameDesc& f, const KParams& p, int arg, uint32_t log2s, hipStream_t stream)
 {
     const uint32_t w = (uint32_t)f.w, h = (uint32_t)f.h;
-    const uint32_t n = STAGE == ST_CAN_ATROUS_PAIR ? pair_tiles_at(w, h, log2s)
+    const uint32_t n = is_pair_stage(STAGE)          ? pair_tiles_at(w, h, log2s)
                        : STAGE == ST_CAN_ATROUS0   ? tall_tiles_at(w, h)
                                                    : tiles_at(w, h, log2s);
-    hipLaunchKernelGGL((k_frame<S, STAGE, COMP>), dim3(n), dim3(BLOCK), 0, stream, f, p, arg, log2s);
+    hipLaunchKernelGGL((k_frame<S, STAGE, SF>), dim3(n), dim3(BLOCK), 0, stream, f, p, arg, log2s);
 }
 
-template <class S, int STAGE, bool COMP>
+template <class S, int STAGE, int SF>
 void launch_batch(const BatchTable& bt, const KParams& p, int arg, uint32_t log2s, int stage, const uint32_t* grids,
                   hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_batch<S, STAGE, COMP>), dim3(grids[stage]), dim3(BLOCK), 0, stream, bt, p, arg, log2s, stage);
+    hipLaunchKernelGGL((k_batch<S, STAGE, SF>), dim3(grids[stage]), dim3(BLOCK), 0, stream, bt, p, arg, log2s, stage);
 }
 
 inline void mark(hipEvent_t* ev, int i, hipStream_t s)
@@ -1100,29 +1187,34 @@ template <class S>
 void reference_frame(const FrameDesc& f, const KParams& p, hipStream_t s, hipEvent_t* ev)
 {
     mark(ev, 0, s);
-    launch_frame<S, ST_REF_INIT, false>(f, p, 0, 0, s);
+    launch_frame<S, ST_REF_INIT, SF_RAW>(f, p, 0, 0, s);
     mark(ev, 1, s);
     for (int pass = 1; pass <= p.iterations; ++pass) {
-        launch_frame<S, ST_REF_ATROUS, false>(f, p, pass, (uint32_t)(pass - 1), s);
+        launch_frame<S, ST_REF_ATROUS, SF_RAW>(f, p, pass, (uint32_t)(pass - 1), s);
         mark(ev, pass + 1, s);
     }
 }
 
-template <class S, bool COMP>
+template <class S, int SF>
 void canonical_frame(const FrameDesc& f, const KParams& p, hipStream_t s, hipEvent_t* ev)
 {
     mark(ev, 0, s);
-    launch_frame<S, ST_CAN_TEMPORAL, COMP>(f, p, 0, 0, s);
+    launch_frame<S, ST_CAN_TEMPORAL, SF>(f, p, 0, 0, s);
     mark(ev, 1, s);
     // A single small frame has few tiles: halving them (tall tiles, residue pairs) would leave
     // the device emptier, so both are used from 2048 16x16 single-residue tiles on.
-    if (tiles_at((uint32_t)f.w, (uint32_t)f.h, 0) < 2048u) launch_frame<S, ST_CAN_ATROUS0_SQ, COMP>(f, p, 0, 0, s);
-    else launch_frame<S, ST_CAN_ATROUS0, COMP>(f, p, 0, 0, s);
+    if (tiles_at((uint32_t)f.w, (uint32_t)f.h, 0) < 2048u) launch_frame<S, ST_CAN_ATROUS0_SQ, SF>(f, p, 0, 0, s);
+    else launch_frame<S, ST_CAN_ATROUS0, SF>(f, p, 0, 0, s);
     mark(ev, 2, s);
     for (int k = 1; k < p.iterations; ++k) {
-        if (k < (int)PAIR_LOG2S || tiles_at((uint32_t)f.w, (uint32_t)f.h, (uint32_t)k) < 2048u)
-            launch_frame<S, ST_CAN_ATROUS, COMP>(f, p, k, (uint32_t)k, s);
-        else launch_frame<S, ST_CAN_ATROUS_PAIR, COMP>(f, p, k, (uint32_t)k, s);
+        const bool last = k == p.iterations - 1;
+        if (k < (int)PAIR_LOG2S || tiles_at((uint32_t)f.w, (uint32_t)f.h, (uint32_t)k) < 2048u) {
+            if (last) launch_frame<S, ST_CAN_ATROUS_L, SF>(f, p, k, (uint32_t)k, s);
+            else launch_frame<S, ST_CAN_ATROUS, SF>(f, p, k, (uint32_t)k, s);
+        } else {
+            if (last) launch_frame<S, ST_CAN_ATROUS_PAIR_L, SF>(f, p, k, (uint32_t)k, s);
+            else launch_frame<S, ST_CAN_ATROUS_PAIR, SF>(f, p, k, (uint32_t)k, s);
+        }
         mark(ev, k + 2, s);
     }
 }
@@ -1131,42 +1223,48 @@ template <class S>
 void reference_batch(const BatchTable& bt, const KParams& p, const uint32_t* g, hipStream_t s, hipEvent_t* ev)
 {
     mark(ev, 0, s);
-    launch_batch<S, ST_REF_INIT, false>(bt, p, 0, 0, 0, g, s);
+    launch_batch<S, ST_REF_INIT, SF_RAW>(bt, p, 0, 0, 0, g, s);
     mark(ev, 1, s);
     for (int pass = 1; pass <= p.iterations; ++pass) {
-        launch_batch<S, ST_REF_ATROUS, false>(bt, p, pass, (uint32_t)(pass - 1), pass, g, s);
+        launch_batch<S, ST_REF_ATROUS, SF_RAW>(bt, p, pass, (uint32_t)(pass - 1), pass, g, s);
         mark(ev, pass + 1, s);
     }
 }
 
-template <class S, bool COMP>
+template <class S, int SF>
 void canonical_batch(const BatchTable& bt, const KParams& p, const uint32_t* g, hipStream_t s, hipEvent_t* ev)
 {
     mark(ev, 0, s);
-    launch_batch<S, ST_CAN_TEMPORAL, COMP>(bt, p, 0, 0, 0, g, s);
+    launch_batch<S, ST_CAN_TEMPORAL, SF>(bt, p, 0, 0, 0, g, s);
     mark(ev, 1, s);
-    launch_batch<S, ST_CAN_ATROUS0, COMP>(bt, p, 0, 0, 1, g, s);
+    launch_batch<S, ST_CAN_ATROUS0, SF>(bt, p, 0, 0, 1, g, s);
     mark(ev, 2, s);
     for (int k = 1; k < p.iterations; ++k) {
-        if (k < (int)PAIR_LOG2S) launch_batch<S, ST_CAN_ATROUS, COMP>(bt, p, k, (uint32_t)k, k + 1, g, s);
-        else launch_batch<S, ST_CAN_ATROUS_PAIR, COMP>(bt, p, k, (uint32_t)k, k + 1, g, s);
+        const bool last = k == p.iterations - 1;
+        if (k < (int)PAIR_LOG2S) {
+            if (last) launch_batch<S, ST_CAN_ATROUS_L, SF>(bt, p, k, (uint32_t)k, k + 1, g, s);
+            else launch_batch<S, ST_CAN_ATROUS, SF>(bt, p, k, (uint32_t)k, k + 1, g, s);
+        } else {
+            if (last) launch_batch<S, ST_CAN_ATROUS_PAIR_L, SF>(bt, p, k, (uint32_t)k, k + 1, g, s);
+            else launch_batch<S, ST_CAN_ATROUS_PAIR, SF>(bt, p, k, (uint32_t)k, k + 1, g, s);
+        }
         mark(ev, k + 2, s);
     }
 }
 
-template <class S, bool CANON, bool COMP>
+template <class S, bool CANON, int SF>
 hipError_t persist_launch(const BatchTable& bt, const KParams& p, PersistCtl* ctl, uint32_t* err, uint64_t max_polls,
                           uint32_t blocks, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_persist<S, CANON, COMP>), dim3(blocks), dim3(BLOCK), 0, s, bt, p, ctl, err, max_polls);
+    hipLaunchKernelGGL((k_persist<S, CANON, SF>), dim3(blocks), dim3(BLOCK), 0, s, bt, p, ctl, err, max_polls);
     return hipGetLastError();
 }
 
-template <class S, bool CANON, bool COMP>
+template <class S, bool CANON, int SF>
 int persist_occupancy()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persist<S, CANON, COMP>, BLOCK, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_persist<S, CANON, SF>, BLOCK, 0) != hipSuccess) return 0;
     return n;
 }
 
@@ -1181,12 +1279,14 @@ hipError_t launch_reference_frame(const FrameDesc& f, const KParams& p, Storage 
 
 hipError_t launch_canonical_frame(const FrameDesc& f, const KParams& p, Storage st, hipStream_t stream, hipEvent_t* ev)
 {
+    const int sf = state_form(p);
     if (st == Storage::F32) {
-        if (p.compensated) canonical_frame<F32S, true>(f, p, stream, ev);
-        else canonical_frame<F32S, false>(f, p, stream, ev);
+        if (sf == SF_COMP) canonical_frame<F32S, SF_COMP>(f, p, stream, ev);
+        else canonical_frame<F32S, SF_RAW>(f, p, stream, ev);
     } else {
-        if (p.compensated) canonical_frame<F16S, true>(f, p, stream, ev);
-        else canonical_frame<F16S, false>(f, p, stream, ev);
+        if (sf == SF_WIDE) canonical_frame<F16S, SF_WIDE>(f, p, stream, ev);
+        else if (sf == SF_COMP) canonical_frame<F16S, SF_COMP>(f, p, stream, ev);
+        else canonical_frame<F16S, SF_RAW>(f, p, stream, ev);
     }
     return hipGetLastError();
 }
@@ -1202,12 +1302,14 @@ hipError_t launch_reference_batch(const BatchTable& bt, const KParams& p, Storag
 hipError_t launch_canonical_batch(const BatchTable& bt, const KParams& p, Storage st, const uint32_t* grid_blocks,
                                   hipStream_t stream, hipEvent_t* ev)
 {
+    const int sf = state_form(p);
     if (st == Storage::F32) {
-        if (p.compensated) canonical_batch<F32S, true>(bt, p, grid_blocks, stream, ev);
-        else canonical_batch<F32S, false>(bt, p, grid_blocks, stream, ev);
+        if (sf == SF_COMP) canonical_batch<F32S, SF_COMP>(bt, p, grid_blocks, stream, ev);
+        else canonical_batch<F32S, SF_RAW>(bt, p, grid_blocks, stream, ev);
     } else {
-        if (p.compensated) canonical_batch<F16S, true>(bt, p, grid_blocks, stream, ev);
-        else canonical_batch<F16S, false>(bt, p, grid_blocks, stream, ev);
+        if (sf == SF_WIDE) canonical_batch<F16S, SF_WIDE>(bt, p, grid_blocks, stream, ev);
+        else if (sf == SF_COMP) canonical_batch<F16S, SF_COMP>(bt, p, grid_blocks, stream, ev);
+        else canonical_batch<F16S, SF_RAW>(bt, p, grid_blocks, stream, ev);
     }
     return hipGetLastError();
 }
@@ -1216,23 +1318,25 @@ hipError_t launch_batch_persistent(int canonical, const BatchTable& bt, const KP
                                    uint32_t* err, uint64_t max_polls, uint32_t blocks, hipStream_t stream)
 {
     if (!canonical) {
-        if (st == Storage::F32) return persist_launch<F32S, false, false>(bt, p, ctl, err, max_polls, blocks, stream);
-        return persist_launch<F16S, false, false>(bt, p, ctl, err, max_polls, blocks, stream);
+        if (st == Storage::F32) return persist_launch<F32S, false, SF_RAW>(bt, p, ctl, err, max_polls, blocks, stream);
+        return persist_launch<F16S, false, SF_RAW>(bt, p, ctl, err, max_polls, blocks, stream);
     }
+    const int sf = state_form(p);
     if (st == Storage::F32) {
-        if (p.compensated) return persist_launch<F32S, true, true>(bt, p, ctl, err, max_polls, blocks, stream);
-        return persist_launch<F32S, true, false>(bt, p, ctl, err, max_polls, blocks, stream);
+        if (sf == SF_COMP) return persist_launch<F32S, true, SF_COMP>(bt, p, ctl, err, max_polls, blocks, stream);
+        return persist_launch<F32S, true, SF_RAW>(bt, p, ctl, err, max_polls, blocks, stream);
     }
-    if (p.compensated) return persist_launch<F16S, true, true>(bt, p, ctl, err, max_polls, blocks, stream);
-    return persist_launch<F16S, true, false>(bt, p, ctl, err, max_polls, blocks, stream);
+    if (sf == SF_WIDE) return persist_launch<F16S, true, SF_WIDE>(bt, p, ctl, err, max_polls, blocks, stream);
+    if (sf == SF_COMP) return persist_launch<F16S, true, SF_COMP>(bt, p, ctl, err, max_polls, blocks, stream);
+    return persist_launch<F16S, true, SF_RAW>(bt, p, ctl, err, max_polls, blocks, stream);
 }
 
-int persistent_blocks_per_cu(int canonical, Storage st, bool compensated)
+int persistent_blocks_per_cu(int canonical, Storage st, int sf)
 {
-    if (!canonical) return st == Storage::F32 ? persist_occupancy<F32S, false, false>() : persist_occupancy<F16S, false, false>();
-    if (st == Storage::F32)
-        return compensated ? persist_occupancy<F32S, true, true>() : persist_occupancy<F32S, true, false>();
-    return compensated ? persist_occupancy<F16S, true, true>() : persist_occupancy<F16S, true, false>();
+    if (!canonical) return st == Storage::F32 ? persist_occupancy<F32S, false, SF_RAW>() : persist_occupancy<F16S, false, SF_RAW>();
+    if (st == Storage::F32) return sf == SF_COMP ? persist_occupancy<F32S, true, SF_COMP>() : persist_occupancy<F32S, true, SF_RAW>();
+    if (sf == SF_WIDE) return persist_occupancy<F16S, true, SF_WIDE>();
+    return sf == SF_COMP ? persist_occupancy<F16S, true, SF_COMP>() : persist_occupancy<F16S, true, SF_RAW>();
 }
 
 }  // namespace svgf

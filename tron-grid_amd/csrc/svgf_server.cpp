@@ -42,6 +42,7 @@ struct Lane {
     hipEvent_t ev = nullptr;
     bool busy = false;
     uint64_t launch_ns = 0;
+    uint64_t batch = 0;  // number of the batch the lane carries (svgf_batch_last)
     svgf_status status = SVGF_OK;
     std::vector<Pending> items;
 };
@@ -128,7 +129,9 @@ void scheduler(svgf_server* s)
                 any_busy = true;
                 continue;
             }
-            if (q != hipSuccess) l.status = SVGF_ERR_HIP;
+            // the batch's own outcome (a persistent launch whose dependency wait gave up
+            // reports SVGF_ERR_TIMEOUT; its streams restart cold)
+            l.status = q != hipSuccess ? SVGF_ERR_HIP : svgf_batch_status(s->ctx, l.batch, 1);
             complete_lane(s, l, now_ns());
             s->cv_done.notify_all();
         }
@@ -149,6 +152,7 @@ void scheduler(svgf_server* s)
             l.items = batch;
             l.launch_ns = now_ns();
             l.status = svgf_denoise_batch(s->ctx, reqs.data(), (uint32_t)reqs.size(), l.stream);
+            if (l.status == SVGF_OK) l.status = svgf_batch_last(s->ctx, &l.batch);
             if (l.status == SVGF_OK && hipEventRecord(l.ev, l.stream) != hipSuccess) l.status = SVGF_ERR_HIP;
             if (l.status != SVGF_OK) {
                 complete_lane(s, l, now_ns());

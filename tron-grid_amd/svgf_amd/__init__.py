@@ -29,9 +29,11 @@ STORAGE_FP32 = 0
 STORAGE_FP16 = 1
 FMT_RGBA32F, FMT_RGBA16F, FMT_RG32F, FMT_RG16F, FMT_R32UI = 1, 2, 3, 4, 5
 FLAG_COMPENSATED = 0x1
+FLAG_WIDE_STATE = 0x2
 BATCH_NO_GRAPH = 0x1
 PROFILE_STAGES = 0x2
 BATCH_PERSISTENT = 0x4
+BATCH_TIMED_STAGES = 0x8
 
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "unsupported", 3: "out of device memory", 4: "over HBM budget",
@@ -223,6 +225,8 @@ def lib():
             "svgf_stream_state_get": [vp, ctypes.c_uint32, P(StreamState)],
             "svgf_budget_used": [vp, P(ctypes.c_uint64)],
             "svgf_denoise_batch": [vp, P(Request), ctypes.c_uint32, vp],
+            "svgf_batch_last": [vp, P(ctypes.c_uint64)],
+            "svgf_batch_status": [vp, ctypes.c_uint64, ctypes.c_int32],
             "svgf_synth_frame": [P(SynthDesc), vp, vp, vp],
             "svgf_stage_times": [vp, P(ctypes.c_float), ctypes.c_uint32, P(ctypes.c_uint32)],
             "svgf_server_create": [vp, P(ServerDesc), P(vp)],
@@ -330,6 +334,45 @@ def fmt2(storage):
     return FMT_RG32F if storage == STORAGE_FP32 else FMT_RG16F
 
 
+def moment_fmt(storage, flags):
+    """Format of the moments planes: the wide state's four binary16 words, or two channels."""
+    return FMT_RGBA16F if flags & FLAG_WIDE_STATE else fmt2(storage)
+
+
+# ---------------------------------------------------------------------------------------------
+# wide FP16 state (svgf.h SVGF_FLAG_WIDE_STATE; csrc/svgf_state.hpp)
+# ---------------------------------------------------------------------------------------------
+def _wide_step(h16):
+    """2^(max(e, 1) - 28) for binary16 values h16 (e = biased exponent)."""
+    e = (np.asarray(h16, np.float16).view(np.uint16).astype(np.int32) >> 10) & 31
+    return np.ldexp(1.0, np.maximum(e, 1) - 28)
+
+
+def decode_history(h):
+    """Wide history plane (h, w, 4) float16 -> (colour (h, w, 3) float64, length (h, w) int)."""
+    h = np.asarray(h, np.float16)
+    word = h[..., 3].view(np.uint16).astype(np.int32)
+    rgb = np.empty(h.shape[:-1] + (3,), np.float64)
+    for c in range(3):
+        q = (word >> (6 + 3 * c)) & 7
+        q = np.where(q >= 4, q - 8, q)
+        v = h[..., c].astype(np.float64)
+        rgb[..., c] = v + np.where(np.isfinite(v), q * _wide_step(h[..., c]), 0.0)
+    return rgb, word & 63
+
+
+def decode_moments(m):
+    """Wide moments plane (h, w, 4) float16 (mean_hi, sigma_hi, mean_lo, sigma_lo) ->
+    (h, w, 2) float64 (mean, sigma): hi + lo 2^(max(e, 1) - 25), e the biased exponent of hi."""
+    m = np.asarray(m, np.float16)
+    out = []
+    for c in range(2):
+        hi = m[..., c]
+        lo = m[..., c + 2].astype(np.float64)
+        out.append(hi.astype(np.float64) + lo * (_wide_step(hi) * 8.0))
+    return np.stack(out, axis=-1)
+
+
 class Context:
     """One svgf_context (replaces the SVGF pipeline + descriptor sets of src/main.cpp:2288-2446)."""
 
@@ -340,6 +383,8 @@ class Context:
         self.params = params if params is not None else default_params(mode, storage)
         self.desc = Desc(mode, storage, self.params, max_width, max_height, max_streams, max_batch_frames,
                          max_batch_pixels, hbm_budget_bytes, device, batch_flags)
+        self.moment_format = moment_fmt(storage, self.params.flags)
+        self.moment_channels = 4 if self.moment_format == FMT_RGBA16F else 2
         h = ctypes.c_void_p()
         check(lib().svgf_create(ctypes.byref(self.desc), ctypes.byref(h)), "svgf_create")
         self.handle = h
@@ -364,6 +409,12 @@ class Context:
     def set_params(self, p: Params):
         check(lib().svgf_set_params(self.handle, ctypes.byref(p)), "svgf_set_params")
         self.params = p
+        self.moment_format = moment_fmt(self.storage, p.flags)
+        self.moment_channels = 4 if self.moment_format == FMT_RGBA16F else 2
+
+    @property
+    def wide(self) -> bool:
+        return bool(self.params.flags & FLAG_WIDE_STATE)
 
     def denoise(self, *, radiance, denoised, moments_curr, moments_prev=None, gbuffer=None, gbuffer_prev=None,
                 motion=None, history_prev=None, history_curr=None, history_valid=True, stream=None):
@@ -374,8 +425,8 @@ class Context:
         f.width, f.height, f.history_valid = w, h, int(bool(history_valid))
         f.radiance = plane_of(radiance, fmt4(st))
         f.denoised = plane_of(denoised, fmt4(st))
-        f.moments_curr = plane_of(moments_curr, fmt2(st))
-        f.moments_prev = plane_of(moments_prev, fmt2(st))
+        f.moments_curr = plane_of(moments_curr, self.moment_format)
+        f.moments_prev = plane_of(moments_prev, self.moment_format)
         f.gbuffer = plane_of(gbuffer, FMT_RGBA32F)
         f.gbuffer_prev = plane_of(gbuffer_prev, FMT_RGBA32F)
         f.motion = plane_of(motion, FMT_RG32F)
@@ -424,6 +475,15 @@ class Context:
             r.gbuffer = plane_of(it.get("gbuffer"), FMT_RGBA32F)
             r.motion = plane_of(it.get("motion"), FMT_RG32F)
         check(lib().svgf_denoise_batch(self.handle, arr, len(items), _stream_handle(stream)), "svgf_denoise_batch")
+
+    def batch_last(self) -> int:
+        b = ctypes.c_uint64()
+        check(lib().svgf_batch_last(self.handle, ctypes.byref(b)), "svgf_batch_last")
+        return b.value
+
+    def batch_status(self, batch: int, wait: bool = True) -> int:
+        """svgf_status of a batch (0 ok, 9 timeout, 10 not ready)."""
+        return lib().svgf_batch_status(self.handle, batch, int(wait))
 
 
 class Server:
