@@ -208,7 +208,10 @@ SVGF_API svgf_status svgf_create(const svgf_desc* desc, svgf_context** out);
 SVGF_API svgf_status svgf_destroy(svgf_context* ctx);
 SVGF_API svgf_status svgf_set_params(svgf_context* ctx, const svgf_params* params);
 
-/* --- single frame (replaces the SVGF block of recordFrame, main.cpp:907-947) ------------ */
+/* --- single frame (replaces the SVGF block of recordFrame, main.cpp:907-947) ------------
+ * The calls of one context share its scratch: a call on another hip_stream than the previous
+ * call first waits, on the device, for that call, so calls on different streams run one after
+ * the other on the GPU (they never race through the scratch). */
 SVGF_API svgf_status svgf_denoise(svgf_context* ctx, const svgf_frame* frame, void* hip_stream);
 
 /* --- resident streams (history reset = main.cpp:1570-1603 resize semantics) ------------- */

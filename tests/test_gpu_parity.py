@@ -372,3 +372,39 @@ def test_timed_stages_inside_graph(S, persistent):
     for f in range(4):
         for i in range(len(shapes)):
             assert torch.equal(a[f][i], b[f][i]), (f, i)
+
+
+@pytest.mark.parametrize("mode", ["reference", "canonical"])
+def test_single_frame_calls_on_two_streams(S, mode):
+    """Two single-frame calls through one context on two HIP streams, enqueued back to back
+    without a host synchronisation, share the context's scratch: each result equals the one the
+    same frame gives alone on one stream, bit for bit (the second call waits on the device for
+    the first)."""
+    dev = torch.device("cuda")
+    w, h = 1280, 720
+    m = S.MODE_REFERENCE if mode == "reference" else S.MODE_CANONICAL
+    dt = torch.float16
+    frames = [S.synth_frame(w, h, seed=30 + k, frame=k, pan_px=0.5, disocclusion=0.05) for k in range(2)]
+    with S.Context(m, S.STORAGE_FP16, max_width=w, max_height=h) as ctx:
+        def planes(d):
+            return {"radiance": torch.from_numpy(d["radiance"]).to(dev).to(dt),
+                    "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev) if m else None,
+                    "history_curr": torch.empty((h, w, 4), dtype=dt, device=dev) if m else None,
+                    "moments_curr": torch.empty((h, w, ctx.moment_channels), dtype=dt, device=dev),
+                    "denoised": torch.empty((h, w, 4), dtype=dt, device=dev), "history_valid": False}
+        alone = []
+        for d in frames:
+            pl = planes(d)
+            ctx.denoise(**pl)
+            torch.cuda.synchronize()
+            alone.append(pl)
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        both = [planes(d) for d in frames]
+        torch.cuda.synchronize()
+        ctx.denoise(stream=s1, **both[0])
+        ctx.denoise(stream=s2, **both[1])
+        torch.cuda.synchronize()
+        for a, b in zip(alone, both):
+            for k in ("denoised", "moments_curr", "history_curr"):
+                if a[k] is not None:
+                    assert torch.equal(a[k].view(torch.int16), b[k].view(torch.int16)), k

@@ -22,7 +22,7 @@ namespace {
 constexpr int BATCH_SLOTS = 3;
 constexpr size_t ALIGN = 256;
 
-inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 uint32_t bytes_of(uint32_t fmt)
 {
@@ -77,6 +77,9 @@ struct svgf_context {
     uint32_t w4 = 16;          // bytes of a wavelet-scratch pixel (FP32 in CANONICAL mode)
 
     void* scratch = nullptr;
+    hipEvent_t scratch_ev = nullptr;     // recorded after every single-frame call
+    hipStream_t scratch_stream = nullptr;  // the stream of the last single-frame call
+    bool scratch_used = false;
 
     std::vector<StreamSlot> streams;
     uint64_t budget_used = 0;
@@ -163,13 +166,15 @@ svgf_status check_plane(const svgf_plane& pl, uint32_t want_fmt, uint32_t w)
 
 PlaneDev dev_plane(const svgf_plane& p) { return PlaneDev{static_cast<char*>(p.ptr), p.pitch_bytes}; }
 
-uint32_t fmt4(const svgf_context* c) { return c->st == Storage::F32 ? SVGF_FMT_RGBA32F : SVGF_FMT_RGBA16F; }
+uint32_t fmt4(Storage st) { return st == Storage::F32 ? SVGF_FMT_RGBA32F : SVGF_FMT_RGBA16F; }
 // moments: two channels at the storage width, or the wide state's four binary16 words
-uint32_t fmt2(const svgf_context* c)
+uint32_t fmt2(Storage st, const KParams& kp)
 {
-    if (c->kp.wide) return SVGF_FMT_RGBA16F;
-    return c->st == Storage::F32 ? SVGF_FMT_RG32F : SVGF_FMT_RG16F;
+    if (kp.wide) return SVGF_FMT_RGBA16F;
+    return st == Storage::F32 ? SVGF_FMT_RG32F : SVGF_FMT_RG16F;
 }
+uint32_t fmt4(const svgf_context* c) { return fmt4(c->st); }
+uint32_t fmt2(const svgf_context* c) { return fmt2(c->st, c->kp); }
 // bytes of a moments pixel
 uint32_t moment_bytes(uint32_t storage, uint32_t flags)
 {
@@ -186,8 +191,8 @@ uint64_t stream_state_bytes(uint32_t mode, uint32_t s4, uint32_t s2, uint32_t w,
 }
 
 // Scratch of one frame in a batch or of the single-frame form: the wavelet ping-pong
-// (storage x4) and the uint8 history length.
-uint64_t frame_scratch_bytes(uint32_t s4, uint64_t px)
+// (storage x4), the uint8 history length and the float2 depth gradient.
+constexpr uint64_t frame_scratch_bytes(uint32_t s4, uint64_t px)
 {
     return 2 * align_up(px * s4, ALIGN) + align_up(px, ALIGN) + align_up(px * 8, ALIGN);
 }
@@ -287,6 +292,7 @@ void destroy_context(svgf_context* c)
         if (s.block) (void)hipFree(s.block);
     for (auto& kv : c->pool) (void)hipFree(kv.second);
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->scratch_ev) (void)hipEventDestroy(c->scratch_ev);
     if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -468,6 +474,57 @@ hipError_t capture_slot(svgf_context* c, BatchSlot& s, const uint32_t* tiles)
     return e;
 }
 
+// Checks the planes of one single-frame call and fills its frame descriptor (the scratch
+// planes are the caller's to carve).
+svgf_status single_frame_desc(uint32_t mode, Storage st, const KParams& kp, const svgf_frame* fr, FrameDesc& f)
+{
+    const uint32_t w = fr->width, h = fr->height;
+    if (w < 1 || h < 1) return SVGF_ERR_INVALID_ARGUMENT;
+    svgf_status s;
+    const bool hv = fr->history_valid != 0;
+    if ((s = check_plane(fr->radiance, fmt4(st), w)) != SVGF_OK) return s;
+    if ((s = check_plane(fr->moments_curr, fmt2(st, kp), w)) != SVGF_OK) return s;
+    if ((s = check_plane(fr->denoised, fmt4(st), w)) != SVGF_OK) return s;
+    if (hv && (s = check_plane(fr->moments_prev, fmt2(st, kp), w)) != SVGF_OK) return s;
+    if (mode == SVGF_MODE_CANONICAL) {
+        if ((s = check_plane(fr->gbuffer, SVGF_FMT_RGBA32F, w)) != SVGF_OK) return s;
+        if ((s = check_plane(fr->history_curr, fmt4(st), w)) != SVGF_OK) return s;
+        if (hv) {
+            if ((s = check_plane(fr->gbuffer_prev, SVGF_FMT_RGBA32F, w)) != SVGF_OK) return s;
+            if ((s = check_plane(fr->motion, SVGF_FMT_RG32F, w)) != SVGF_OK) return s;
+            if ((s = check_plane(fr->history_prev, fmt4(st), w)) != SVGF_OK) return s;
+        }
+    }
+    for (const svgf_plane* pl : {&fr->radiance, &fr->gbuffer, &fr->gbuffer_prev, &fr->motion, &fr->history_prev,
+                                 &fr->history_curr, &fr->moments_prev, &fr->moments_curr, &fr->denoised})
+        if (pl->ptr && !plane_fits(*pl, h)) return SVGF_ERR_CAPACITY;
+    f = FrameDesc{};
+    f.w = (int32_t)w;
+    f.h = (int32_t)h;
+    f.history_valid = hv ? 1 : 0;
+    f.rad = dev_plane(fr->radiance);
+    f.mom_curr = dev_plane(fr->moments_curr);
+    f.den = dev_plane(fr->denoised);
+    if (hv) f.mom_prev = dev_plane(fr->moments_prev);
+    if (mode == SVGF_MODE_CANONICAL) {
+        f.g = dev_plane(fr->gbuffer);
+        f.hist_curr = dev_plane(fr->history_curr);
+        if (hv) {
+            f.gprev = dev_plane(fr->gbuffer_prev);
+            f.mv = dev_plane(fr->motion);
+            f.hist_prev = dev_plane(fr->history_prev);
+        }
+    }
+    return SVGF_OK;
+}
+
+hipError_t launch_single_frame(uint32_t mode, Storage st, const KParams& kp, const FrameDesc& f, hipStream_t stream,
+                               hipEvent_t* ev)
+{
+    return mode == SVGF_MODE_REFERENCE ? launch_reference_frame(f, kp, st, stream, ev)
+                                       : launch_canonical_frame(f, kp, st, stream, ev);
+}
+
 }  // namespace
 
 extern "C" {
@@ -595,6 +652,11 @@ svgf_status svgf_create(const svgf_desc* d, svgf_context** out)
         destroy_context(c);
         return SVGF_ERR_OUT_OF_MEMORY;
     }
+    if (hipEventCreateWithFlags(&c->scratch_ev, hipEventDisableTiming) != hipSuccess) {
+        c->scratch_ev = nullptr;
+        destroy_context(c);
+        return SVGF_ERR_HIP;
+    }
     if (d->batch_flags & SVGF_PROFILE_STAGES) {
         for (auto& e : c->ev)
             if (hipEventCreate(&e) != hipSuccess) {
@@ -682,49 +744,23 @@ svgf_status svgf_denoise(svgf_context* c, const svgf_frame* fr, void* hip_stream
     const uint32_t w = fr->width, h = fr->height;
     if (w < 1 || h < 1) return SVGF_ERR_INVALID_ARGUMENT;
     if (w > c->desc.max_width || h > c->desc.max_height) return SVGF_ERR_CAPACITY;
-    svgf_status s;
-    const bool hv = fr->history_valid != 0;
-    if ((s = check_plane(fr->radiance, fmt4(c), w)) != SVGF_OK) return s;
-    if ((s = check_plane(fr->moments_curr, fmt2(c), w)) != SVGF_OK) return s;
-    if ((s = check_plane(fr->denoised, fmt4(c), w)) != SVGF_OK) return s;
-    if (hv && (s = check_plane(fr->moments_prev, fmt2(c), w)) != SVGF_OK) return s;
-    if (c->mode == SVGF_MODE_CANONICAL) {
-        if ((s = check_plane(fr->gbuffer, SVGF_FMT_RGBA32F, w)) != SVGF_OK) return s;
-        if ((s = check_plane(fr->history_curr, fmt4(c), w)) != SVGF_OK) return s;
-        if (hv) {
-            if ((s = check_plane(fr->gbuffer_prev, SVGF_FMT_RGBA32F, w)) != SVGF_OK) return s;
-            if ((s = check_plane(fr->motion, SVGF_FMT_RG32F, w)) != SVGF_OK) return s;
-            if ((s = check_plane(fr->history_prev, fmt4(c), w)) != SVGF_OK) return s;
-        }
-    }
-    for (const svgf_plane* pl : {&fr->radiance, &fr->gbuffer, &fr->gbuffer_prev, &fr->motion, &fr->history_prev,
-                                 &fr->history_curr, &fr->moments_prev, &fr->moments_curr, &fr->denoised})
-        if (pl->ptr && !plane_fits(*pl, h)) return SVGF_ERR_CAPACITY;
+    FrameDesc f;
+    svgf_status s = single_frame_desc(c->desc.mode, c->st, c->kp, fr, f);
+    if (s != SVGF_OK) return s;
     DeviceGuard guard(c->desc.device);
-    FrameDesc f{};
-    f.w = (int32_t)w;
-    f.h = (int32_t)h;
-    f.history_valid = hv ? 1 : 0;
-    f.rad = dev_plane(fr->radiance);
-    f.mom_curr = dev_plane(fr->moments_curr);
-    f.den = dev_plane(fr->denoised);
-    if (hv) f.mom_prev = dev_plane(fr->moments_prev);
-    if (c->mode == SVGF_MODE_CANONICAL) {
-        f.g = dev_plane(fr->gbuffer);
-        f.hist_curr = dev_plane(fr->history_curr);
-        if (hv) {
-            f.gprev = dev_plane(fr->gbuffer_prev);
-            f.mv = dev_plane(fr->motion);
-            f.hist_prev = dev_plane(fr->history_prev);
-        }
-    }
     // Scratch planes are packed at this frame's width.
     carve_scratch(static_cast<char*>(c->scratch), c->w4, w, h, f);
     hipStream_t st = static_cast<hipStream_t>(hip_stream);
-    hipEvent_t* ev = profile_events(c);
-    hipError_t e = c->mode == SVGF_MODE_REFERENCE ? launch_reference_frame(f, c->kp, c->st, st, ev)
-                                                  : launch_canonical_frame(f, c->kp, c->st, st, ev);
-    return e == hipSuccess ? SVGF_OK : SVGF_ERR_HIP;
+    // Every call uses the context's one scratch: a call on another stream than the previous
+    // call's first waits (on the device) for that call, so calls never overlap through it.
+    if (c->scratch_used && st != c->scratch_stream && hipStreamWaitEvent(st, c->scratch_ev, 0) != hipSuccess)
+        return SVGF_ERR_HIP;
+    hipError_t e = launch_single_frame(c->desc.mode, c->st, c->kp, f, st, profile_events(c));
+    if (e == hipSuccess) e = hipEventRecord(c->scratch_ev, st);
+    if (e != hipSuccess) return SVGF_ERR_HIP;
+    c->scratch_used = true;
+    c->scratch_stream = st;
+    return SVGF_OK;
 }
 
 svgf_status svgf_stream_open(svgf_context* c, uint32_t w, uint32_t h, uint32_t* id)

@@ -373,6 +373,24 @@ def decode_moments(m):
     return np.stack(out, axis=-1)
 
 
+def make_frame(storage, moment_format, *, radiance, denoised, moments_curr, moments_prev=None, gbuffer=None,
+               gbuffer_prev=None, motion=None, history_prev=None, history_curr=None, history_valid=True) -> Frame:
+    """svgf_frame of device tensors (h, w, c)."""
+    f = Frame()
+    h, w = radiance.shape[:2]
+    f.width, f.height, f.history_valid = w, h, int(bool(history_valid))
+    f.radiance = plane_of(radiance, fmt4(storage))
+    f.denoised = plane_of(denoised, fmt4(storage))
+    f.moments_curr = plane_of(moments_curr, moment_format)
+    f.moments_prev = plane_of(moments_prev, moment_format)
+    f.gbuffer = plane_of(gbuffer, FMT_RGBA32F)
+    f.gbuffer_prev = plane_of(gbuffer_prev, FMT_RGBA32F)
+    f.motion = plane_of(motion, FMT_RG32F)
+    f.history_prev = plane_of(history_prev, fmt4(storage))
+    f.history_curr = plane_of(history_curr, fmt4(storage))
+    return f
+
+
 class Context:
     """One svgf_context (replaces the SVGF pipeline + descriptor sets of src/main.cpp:2288-2446)."""
 
@@ -416,22 +434,11 @@ class Context:
     def wide(self) -> bool:
         return bool(self.params.flags & FLAG_WIDE_STATE)
 
-    def denoise(self, *, radiance, denoised, moments_curr, moments_prev=None, gbuffer=None, gbuffer_prev=None,
-                motion=None, history_prev=None, history_curr=None, history_valid=True, stream=None):
-        """Enqueue one frame on `stream` (a torch.cuda.Stream or raw handle; default: current)."""
-        st = self.storage
-        f = Frame()
-        h, w = radiance.shape[:2]
-        f.width, f.height, f.history_valid = w, h, int(bool(history_valid))
-        f.radiance = plane_of(radiance, fmt4(st))
-        f.denoised = plane_of(denoised, fmt4(st))
-        f.moments_curr = plane_of(moments_curr, self.moment_format)
-        f.moments_prev = plane_of(moments_prev, self.moment_format)
-        f.gbuffer = plane_of(gbuffer, FMT_RGBA32F)
-        f.gbuffer_prev = plane_of(gbuffer_prev, FMT_RGBA32F)
-        f.motion = plane_of(motion, FMT_RG32F)
-        f.history_prev = plane_of(history_prev, fmt4(st))
-        f.history_curr = plane_of(history_curr, fmt4(st))
+    def denoise(self, *, stream=None, **planes):
+        """Enqueue one frame on `stream` (a torch.cuda.Stream or raw handle; default: current).
+        planes: radiance, denoised, moments_curr, moments_prev, gbuffer, gbuffer_prev, motion,
+        history_prev, history_curr (tensors or None) and history_valid."""
+        f = make_frame(self.storage, self.moment_format, **planes)
         check(lib().svgf_denoise(self.handle, ctypes.byref(f), _stream_handle(stream)), "svgf_denoise")
 
     # -- resident streams / batched form --
