@@ -214,6 +214,21 @@ SVGF_API svgf_status svgf_set_params(svgf_context* ctx, const svgf_params* param
  * the other on the GPU (they never race through the scratch). */
 SVGF_API svgf_status svgf_denoise(svgf_context* ctx, const svgf_frame* frame, void* hip_stream);
 
+/* --- single frame without allocation --------------------------------------------------------
+ * No counterpart in the reference, whose pass always runs inside an allocated pipeline. For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): svgf_denoise without a context, on the current HIP device,
+ * with mode, storage and parameters given per call. It calls no hipMalloc, hipFree or other
+ * allocating routine: its scratch is a fixed device array of the library holding one frame of
+ * up to SVGF_STATIC_MAX_PIXELS pixels (41 B per pixel, about 38 MB, on each GPU the library runs
+ * on, until the process ends); a larger frame is SVGF_ERR_CAPACITY. Results equal svgf_denoise's
+ * bit for bit. Every call on a GPU uses that GPU's one array and nothing in it needs clearing:
+ * calls that overlap on one GPU (other streams, other threads) race and give wrong results, so
+ * order them on one stream. */
+#define SVGF_STATIC_MAX_PIXELS (1280u * 720u)
+SVGF_API svgf_status svgf_denoise_static(uint32_t mode, uint32_t storage, const svgf_params* params,
+                                         const svgf_frame* frame, void* hip_stream);
+
 /* --- resident streams (history reset = main.cpp:1570-1603 resize semantics) ------------- */
 SVGF_API svgf_status svgf_stream_open(svgf_context* ctx, uint32_t width, uint32_t height, uint32_t* stream_id);
 SVGF_API svgf_status svgf_stream_close(svgf_context* ctx, uint32_t stream_id);

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -527,6 +528,48 @@ hipError_t launch_single_frame(uint32_t mode, Storage st, const KParams& kp, con
 
 }  // namespace
 
+// Scratch of svgf_denoise_static: the single-frame scratch (wavelet ping-pong at FP32 width,
+// history length, depth gradient) of a frame of up to SVGF_STATIC_MAX_PIXELS pixels, in a
+// fixed device array of the library, so that the call allocates nothing. Each GPU the library
+// runs on holds its own copy at its own address until the process ends. Host code never uses
+// the array's name as an address: the per-device address comes from hipGetSymbolAddress.
+constexpr uint64_t STATIC_SCRATCH_BYTES = frame_scratch_bytes(16, SVGF_STATIC_MAX_PIXELS);
+static __device__ __align__(256) char scratch[STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int STATIC_MAX_DEVICES = 64;
+struct StaticDevice {
+    void* ptr = nullptr;  // address of `scratch` on the device
+    int state = 0;        // 0: not set up, 1: ready, < 0: svgf_status of the failed set-up
+};
+std::mutex g_static_mu;
+StaticDevice g_static[STATIC_MAX_DEVICES];
+
+// Set-up of the current device, once: gfx950 check and the array's device address.
+svgf_status static_device(int dev, void** ptr)
+{
+    if (dev < 0 || dev >= STATIC_MAX_DEVICES) return SVGF_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    StaticDevice& d = g_static[dev];
+    if (d.state == 0) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            d.state = -(int)SVGF_ERR_NO_DEVICE;
+        } else if (hipGetSymbolAddress(&d.ptr, HIP_SYMBOL(scratch)) != hipSuccess || !d.ptr) {
+            d.ptr = nullptr;
+            d.state = -(int)SVGF_ERR_HIP;
+        } else {
+            d.state = 1;
+        }
+    }
+    if (d.state < 0) return (svgf_status)(-d.state);
+    *ptr = d.ptr;
+    return SVGF_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 uint32_t svgf_abi_version(void) { return SVGF_ABI_VERSION; }
@@ -761,6 +804,30 @@ svgf_status svgf_denoise(svgf_context* c, const svgf_frame* fr, void* hip_stream
     c->scratch_used = true;
     c->scratch_stream = st;
     return SVGF_OK;
+}
+
+svgf_status svgf_denoise_static(uint32_t mode, uint32_t storage, const svgf_params* params, const svgf_frame* fr,
+                                void* hip_stream)
+{
+    if (!params || !fr || mode > SVGF_MODE_CANONICAL || storage > SVGF_STORAGE_FP16) return SVGF_ERR_INVALID_ARGUMENT;
+    svgf_status s = validate_params(*params, mode, storage);
+    if (s != SVGF_OK) return s;
+    const uint32_t w = fr->width, h = fr->height;
+    if (w < 1 || h < 1) return SVGF_ERR_INVALID_ARGUMENT;
+    if ((uint64_t)w * h > SVGF_STATIC_MAX_PIXELS) return SVGF_ERR_CAPACITY;
+    const Storage st = storage == SVGF_STORAGE_FP32 ? Storage::F32 : Storage::F16;
+    const KParams kp = to_kparams(*params);
+    FrameDesc f;
+    if ((s = single_frame_desc(mode, st, kp, fr, f)) != SVGF_OK) return s;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return SVGF_ERR_NO_DEVICE;
+    void* base = nullptr;
+    if ((s = static_device(dev, &base)) != SVGF_OK) return s;
+    // every plane of the scratch is written (temporal / init stage) before it is read, so
+    // nothing needs clearing; the scratch planes are packed at this frame's width
+    carve_scratch(static_cast<char*>(base), mode == SVGF_MODE_CANONICAL ? 16u : (st == Storage::F32 ? 16u : 8u), w, h, f);
+    const hipError_t e = launch_single_frame(mode, st, kp, f, static_cast<hipStream_t>(hip_stream), nullptr);
+    return e == hipSuccess ? SVGF_OK : SVGF_ERR_HIP;
 }
 
 svgf_status svgf_stream_open(svgf_context* c, uint32_t w, uint32_t h, uint32_t* id)

@@ -219,6 +219,7 @@ def lib():
             "svgf_destroy": [vp],
             "svgf_set_params": [vp, P(Params)],
             "svgf_denoise": [vp, P(Frame), vp],
+            "svgf_denoise_static": [ctypes.c_uint32, ctypes.c_uint32, P(Params), P(Frame), vp],
             "svgf_stream_open": [vp, ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_uint32)],
             "svgf_stream_close": [vp, ctypes.c_uint32],
             "svgf_stream_reset": [vp, ctypes.c_uint32, vp],
@@ -389,6 +390,17 @@ def make_frame(storage, moment_format, *, radiance, denoised, moments_curr, mome
     f.history_prev = plane_of(history_prev, fmt4(storage))
     f.history_curr = plane_of(history_curr, fmt4(storage))
     return f
+
+
+STATIC_MAX_PIXELS = 1280 * 720
+
+
+def denoise_static(mode, storage, params=None, *, stream=None, **planes) -> int:
+    """svgf_denoise_static (no context, no allocation) on the current device; returns the
+    svgf_status (0 on success) instead of raising, so callers can see each error code."""
+    p = params if params is not None else default_params(mode, storage)
+    f = make_frame(storage, moment_fmt(storage, p.flags), **planes)
+    return lib().svgf_denoise_static(mode, storage, ctypes.byref(p), ctypes.byref(f), _stream_handle(stream))
 
 
 class Context:
