@@ -81,6 +81,48 @@ def test_reference_fp16_storage(S, oracle_lib):
         assert mean <= 1e-5, name
 
 
+def test_reference_fp16_moment_overflow_and_nan_taps(S, oracle_lib):
+    """The reference's own FP16 hazards, against the restatement that rounds every store to half:
+    luminance above 256 makes the rg16f second moment overflow to +inf (then NaN through the
+    lerp a + t (b - a) on the next frame, and variance max(NaN - mu^2, 0) = 0); a NaN radiance
+    tap makes the weight sum NaN, so `weight_sum > 1e-6` is false and the pixel keeps its
+    centre colour (svgf_denoise.slang:184-189, 256; main.cpp:1367-1370). 3 frames: every output
+    within one half ulp, non-finite values in the same places, moments bit for bit."""
+    O = oracle_lib
+    w, h = 320, 180
+    rng = np.random.default_rng(9)
+    seq = []
+    for f in range(3):
+        L = S.synth_frame(w, h, seed=40 + f, frame=f)["radiance"].copy()
+        L[40:120, 60:200, :3] *= 200.0  # luminance up to ~2000: mu2 = 0.2 l^2 overflows binary16
+        if f == 1:
+            ys, xs = rng.integers(0, h, 25), rng.integers(0, w, 25)
+            L[ys, xs, rng.integers(0, 3, 25)] = np.nan
+        seq.append(O.round_f16(L))
+    op = O.reference_params(3)
+    dev = torch.device("cuda")
+    mom = [torch.zeros((h, w, 2), dtype=torch.float16, device=dev) for _ in range(2)]
+    om = None
+    with S.Context(S.MODE_REFERENCE, S.STORAGE_FP16, max_width=w, max_height=h) as ctx:
+        for f, L in enumerate(seq):
+            den = torch.empty((h, w, 4), dtype=torch.float16, device=dev)
+            ctx.denoise(radiance=torch.from_numpy(L).to(dev).half(), denoised=den, moments_curr=mom[f % 2],
+                        moments_prev=mom[(f + 1) % 2] if f else None, history_valid=f > 0)
+            torch.cuda.synchronize()
+            om, od, _, _ = O.r_frame_f32(L, om, op, store=O.STORE_F16, history_valid=f > 0)
+            g = den.float().cpu().numpy()
+            gm = mom[f % 2].float().cpu().numpy()
+            assert np.isinf(om[..., 1]).any(), f  # the hazard is exercised
+            if f == 2:
+                assert np.isnan(om[..., 1]).any()
+            assert np.array_equal(gm, om, equal_nan=True), f
+            fin = np.isfinite(od)
+            assert np.array_equal(np.isfinite(g), fin) and np.array_equal(np.isnan(g), np.isnan(od)), f
+            mx, _ = rel_err(g[fin], od[fin])
+            print(f"frame {f}: {int((~fin).sum())} non-finite outputs, max rel err {mx:.3e}")
+            assert mx <= 2.0 ** -10 + 1e-6, f
+
+
 @pytest.mark.parametrize("shape", [(1, 1), (2, 9), (9, 2), (65, 5), (67, 131)])
 def test_reference_edge_sizes(S, oracle_lib, shape):
     O = oracle_lib
