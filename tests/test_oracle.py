@@ -122,6 +122,77 @@ def test_equal_luma_reduces_to_b3_spline_blur():
     np.testing.assert_allclose(den[..., :3], want, rtol=1e-9, atol=1e-12)
 
 
+def test_reference_luma_edge_stop_on_a_two_level_step():
+    """A grey step (luminance la left of column 6, lb from it) through the cold first frame and
+    one a-trous pass: every tap on the centre's side weighs K[dx], every tap across weighs
+    K[dx] r with r = exp(-|lb - la| / max(2 sigma_c, eps)) and sigma_c = sqrt(0.16 l_c^2 + eps)
+    the centre's (cold moments: var = alpha (1 - alpha) l^2). So the four pixels next to the
+    step are (S_same l_c + S_cross r l_o) / (S_same + S_cross r) with S = 15/16, 1/16 or 11/16,
+    5/16, and all others keep their level. Fixes the sign, the factor 2 and the epsilon of the
+    edge stop, the centre's (not the tap's) sigma, and the normalisation."""
+    w, h, edge = 12, 3, 6
+    a, b = 1.0, 4.0
+    L = np.zeros((h, w, 4))
+    L[:, :edge, :3] = a
+    L[:, edge:, :3] = b
+    L[..., 3] = 1.0
+    p = O.reference_params(1)
+    _, den = O.r_frame_f64(L, None, p, history_valid=False)
+    la, lb = luma(L[0, 0]), luma(L[0, -1])
+    alpha, eps = float(np.float32(0.2)), float(np.float32(1e-4))
+
+    def r(lc):
+        sig = math.sqrt(alpha * (1 - alpha) * lc * lc + eps)
+        return math.exp(-abs(lb - la) / max(2.0 * sig, eps))
+
+    want = np.where(np.arange(w) < edge, a, b).astype(np.float64)
+    for x, same, cross, lc, mine, other in ((4, 15 / 16, 1 / 16, la, a, b), (5, 11 / 16, 5 / 16, la, a, b),
+                                            (6, 11 / 16, 5 / 16, lb, b, a), (7, 15 / 16, 1 / 16, lb, b, a)):
+        want[x] = (same * mine + cross * r(lc) * other) / (same + cross * r(lc))
+    for y in range(h):
+        for c in range(3):
+            np.testing.assert_allclose(den[y, :, c], want, rtol=1e-12)
+
+
+def test_canonical_depth_and_normal_stops_are_separable_filters():
+    """Constant luminance (any chroma), a depth ramp z = 4 + x / 128 and normals turning by
+    theta per column, every row alike: then |z_q - z_p| = |dx| s / 128, the central-difference
+    gradient is (1/128, 0) and n_p.n_q = cos(theta dx s), so a-trous iteration k is the fixed
+    separable filter K[dy] x K[dx] exp(-|dx| s / (128 (phi_z |dx| s / 128 + eps_z)))
+    cos(theta |dx| s)^phi_n, s = 2^k: the whole pass is five normalised 1-D convolutions along
+    the rows (numpy). Compared away from the image edges. Fixes the depth term's gradient,
+    stride, phi_z and epsilon, and the normal term's exponent."""
+    w, h = 256, 4
+    x = np.arange(w)
+    theta = 0.003
+    r = 1.0 + 0.5 * np.sin(0.3 * x)
+    bb = 1.0 + 0.5 * np.cos(0.17 * x)
+    g = (1.5 - LR * r - LB * bb) / LG
+    rad = np.zeros((h, w, 4), np.float32)
+    rad[..., 0], rad[..., 1], rad[..., 2], rad[..., 3] = r, g, bb, 1.0
+    G = np.zeros((h, w, 4), np.float32)
+    G[..., 0], G[..., 2], G[..., 3] = np.sin(theta * x), np.cos(theta * x), 4.0 + x / 128.0
+    p = O.canonical_params(5)
+    out = O.c_frame_f64(rad, G, None, np.zeros((h, w, 2), np.float32), None, None, p, history_valid=False)
+    phi_z, eps_z, phi_n = float(p.phi_depth), float(p.depth_eps), float(p.phi_normal)
+    c = rad[0, :, :3].astype(np.float64)
+    for k in range(5):
+        s = 1 << k
+        wts = []
+        for dx in range(-2, 3):
+            t = abs(dx) * s / 128.0
+            wts.append(K5[dx + 2] * math.exp(-t / (phi_z * t + eps_z)) * max(0.0, math.cos(theta * abs(dx) * s)) ** phi_n)
+        c = sum(wt * c[np.clip(x + dx * s, 0, w - 1)] for wt, dx in zip(wts, range(-2, 3))) / sum(wts)
+    inner = slice(64, 192)
+    for y in range(h):
+        np.testing.assert_allclose(out["denoised"][y, inner, :3], c[inner], rtol=1e-4)
+    # the depth and normal terms matter here: a plain B3 blur differs by far more than the bound
+    plain = rad[0, :, :3].astype(np.float64)
+    for k in range(5):
+        plain = _b3_blur_clamped(plain[None], 1 << k)[0]
+    assert np.abs(plain[inner] - c[inner]).max() > 1e-2
+
+
 def test_phi_zero_is_identity_where_luma_differs():
     """sigma -> 0 kills every tap whose luminance differs from the centre; interior pixels
     with distinct luminances come back unchanged."""
