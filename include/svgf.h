@@ -348,7 +348,8 @@ typedef struct svgf_synth_desc {
     uint64_t seed;          /* per-frame noise seed */
     uint64_t stream;        /* agent-stream key */
     uint32_t frame;         /* frame index within the sequence (camera position) */
-    uint32_t noise;         /* 0 = exponential (mean 1), 1 = lognormal (sigma 1.5, mean 1) */
+    uint32_t noise;         /* 0 = exponential (mean 1), 1 = lognormal (sigma 1.5, mean 1),
+                               2 = none (the noise-free radiance, the mean of both) */
     float pan_px;           /* camera pan per frame in pixels (motion vectors) */
     float disocclusion;     /* fraction of pixels whose G-buffer is forced inconsistent */
     uint32_t threads;       /* host threads; 0 = hardware concurrency */

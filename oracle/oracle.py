@@ -19,6 +19,8 @@ LIB_PATH = HERE / "_build" / "libsvgf_oracle.so"
 
 FLAG_COMPENSATED = 0x1
 FLAG_AS_REFERENCE = 0x100
+FLAG_PREFILTER_UNIT = 0x200
+FLAG_EPS_OUTSIDE = 0x400
 STORE_F32 = 0
 STORE_F16 = 1
 

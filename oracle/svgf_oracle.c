@@ -559,12 +559,16 @@ static void c_atrous_rows(const c_job* J)
             } else {
                 double gv = 0.0;
                 static const double K3[3] = {0.25, 0.5, 0.25};
+                const int ps = (p->flags & ORACLE_FLAG_PREFILTER_UNIT) ? 1 : s;
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
-                        int qx = clampi(x + dx * s, 0, w - 1), qy = clampi(y + dy * s, 0, h - 1);
+                        int qx = clampi(x + dx * ps, 0, w - 1), qy = clampi(y + dy * ps, 0, h - 1);
                         gv += K3[dx + 1] * K3[dy + 1] * src[4 * ((size_t)qy * w + qx) + 3];
                     }
-                sigma = (double)p->phi_luma * sqrt(fmax(gv, 0.0) + (double)p->variance_eps);
+                if (p->flags & ORACLE_FLAG_EPS_OUTSIDE)
+                    sigma = (double)p->phi_luma * sqrt(fmax(gv, 0.0)) + (double)p->variance_eps;
+                else
+                    sigma = (double)p->phi_luma * sqrt(fmax(gv, 0.0) + (double)p->variance_eps);
                 Gp = gbuffer + 4 * i;
                 depth_gradient(gbuffer, w, h, x, y, &gx, &gy);
             }

@@ -45,6 +45,12 @@ typedef struct oracle_params {
 #define ORACLE_FLAG_COMPENSATED 0x1u /* moments as (mean, sqrt(centred second moment)) */
 #define ORACLE_FLAG_AS_REFERENCE 0x100u /* oracle_c_* with every canonical feature switched
                                            off: must reproduce oracle_r_f64 */
+/* Schied et al. 2017's forms where CANONICAL deviates (for measuring the deviations only):
+ * the 3x3 variance prefilter over the pixel's direct neighbours (+-1) instead of its lattice
+ * neighbours at +-s, and the luminance scale phi_l sqrt(g(var)) + eps instead of
+ * phi_l sqrt(g(var) + eps). */
+#define ORACLE_FLAG_PREFILTER_UNIT 0x200u
+#define ORACLE_FLAG_EPS_OUTSIDE 0x400u
 
 enum { ORACLE_STORE_F32 = 0, ORACLE_STORE_F16 = 1 };
 

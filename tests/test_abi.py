@@ -177,6 +177,20 @@ def test_synth_lognormal_mean_one_and_fp16_range():
     assert r.max() < 65504.0
 
 
+def test_synth_noise_free_mode_is_the_mean_of_the_noisy_modes():
+    """noise = 2 gives the noise-free radiance: independent of the noise seed, and the mean the
+    two noisy modes scatter around (both noises have mean 1)."""
+    a = S.synth_frame(256, 256, seed=1, frame=2, noise=2, pan_px=0.5)["radiance"]
+    b = S.synth_frame(256, 256, seed=9, frame=2, noise=2, pan_px=0.5)["radiance"]
+    assert np.array_equal(a, b)
+    for noise in (0, 1):
+        n = S.synth_frame(256, 256, seed=1, frame=2, noise=noise, pan_px=0.5)["radiance"]
+        ratio = n[..., :3].astype(np.float64).sum() / a[..., :3].astype(np.float64).sum()
+        assert 0.9 < ratio < 1.1, (noise, ratio)
+    d = S.SynthDesc(8, 8, 0, 0, 0, 3, 0.5, 0.0, 1, 0)
+    assert S.lib().svgf_synth_frame(ctypes.byref(d), None, None, None) == 1
+
+
 def test_synth_pan_motion_is_consistent():
     """Frame f at x shows what frame f-1 shows at x + pan (integer pan: exact)."""
     a = S.synth_frame(64, 16, seed=0, frame=3, pan_px=2.0)

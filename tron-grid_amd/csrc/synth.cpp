@@ -9,7 +9,8 @@
 // u = x + f * pan_px, v = y, and the previous position of pixel x is x + pan_px (motion =
 // (+pan_px, 0)). Depth is a smooth field in [0.1, 100] m, normals come from a procedural
 // heightfield, radiance is a smooth 0-10 gradient (tinted) times per-pixel noise with mean 1:
-// exponential (1 spp) or lognormal (sigma 1.5, mu -1.125). A `disocclusion` fraction of the
+// exponential (1 spp) or lognormal (sigma 1.5, mu -1.125), or none (the noise-free radiance
+// both have as their mean: the ground truth of a denoised sequence). A `disocclusion` fraction of the
 // pixels gets its depth scaled by 1.5, which the history depth test rejects.
 #include "../../include/svgf.h"
 
@@ -76,6 +77,7 @@ struct Gen {
     double noise(uint64_t pixel) const
     {
         uint64_t h1 = key(d->seed, d->stream, d->frame, pixel, 1);
+        if (d->noise == 2) return 1.0;
         if (d->noise == 0) return -std::log(u01(h1));
         uint64_t h2 = key(d->seed, d->stream, d->frame, pixel, 2);
         double nrm = std::sqrt(-2.0 * std::log(u01(h1))) * std::cos(2.0 * PI * u01(h2));
@@ -122,7 +124,7 @@ void rows(const Gen& g, uint32_t y0, uint32_t y1, float* rad, float* gb, float* 
 
 extern "C" svgf_status svgf_synth_frame(const svgf_synth_desc* d, float* radiance, float* gbuffer, float* motion)
 {
-    if (!d || d->width < 1 || d->height < 1 || d->noise > 1 || !(d->disocclusion >= 0.0f && d->disocclusion <= 1.0f))
+    if (!d || d->width < 1 || d->height < 1 || d->noise > 2 || !(d->disocclusion >= 0.0f && d->disocclusion <= 1.0f))
         return SVGF_ERR_INVALID_ARGUMENT;
     Gen g{d, (double)d->width, (double)d->height};
     uint32_t nt = d->threads ? d->threads : std::max(1u, std::thread::hardware_concurrency());
