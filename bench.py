@@ -932,6 +932,11 @@ def main():
                          if head == "persistent" else
                          "batched canonical SVGF, 64 agent frames per launch sequence (one launch per stage), hipGraph"),
             "launch": head,
+            "launch_note": ("one kernel per stage over the ragged tiles of all 64 frames, replayed from a hipGraph; "
+                            "the single persistent launch per batch (batched_other_launch) is about 2x slower: every "
+                            "workgroup pays a device-scope release and acquire (L2 write-back / invalidate) at each of "
+                            "the 6 stage changes, where a kernel boundary pays it once per XCD"
+                            if head == "per_stage" else "one persistent launch per batch"),
             "frames_per_batch": args.frames,
             "pixels_per_batch": batch_px,
             "resolutions": "uniform over 96x96,128x128,256x256,320x180,640x480,1280x720 (seed 0)",
@@ -964,6 +969,22 @@ def main():
         "batched_without_stage_events": plain_graph,
         "plain_fp32": {"value": round(plain_mpx, 2), "unit": "Mpx/s", "bytes_per_px": bytes_px_32,
                        "ratio_batched_over_plain": round(batched_mpx / plain_mpx, 3)},
+        # the stated targets of the batched FP16 form and where this run stands
+        "targets": {
+            "ratio_over_plain_fp32": {"target": 3.0, "measured": round(batched_mpx / plain_mpx, 3),
+                                      "met": batched_mpx / plain_mpx >= 3.0},
+            "hbm_peak_fraction": {"target": 0.6,
+                                  "measured": round(batched_mpx * 1e6 * SURVEY_FP16_FRAME_BYTES_PX / 1e9 / HBM_PEAK_GBPS, 4),
+                                  "measured_layout_bytes": round(batched_mpx * 1e6 * bytes_px_16 / 1e9 / HBM_PEAK_GBPS, 4),
+                                  "met": batched_mpx * 1e6 * SURVEY_FP16_FRAME_BYTES_PX / 1e9 / HBM_PEAK_GBPS >= 0.6},
+            "max_rel_err_vs_fp64": {"target": 1e-3,
+                                    "measured": None if accuracy is None else max(accuracy["max_rel_err"].values()),
+                                    "met": None if accuracy is None else accuracy["within_bound"]},
+            "p99_reported": True,
+            "why": "the a-trous iterations issue ~16 VALU instructions per tap (2 transcendental) for 25 taps a "
+                   "pixel, at 75-83 % VALU issue: arithmetic, not HBM, bounds every stage but the temporal one, "
+                   "and faster kernels speed the plain FP32 form as much as the batched one",
+        },
         "effective_GBps": round(batched_mpx * 1e6 * bytes_px_16 / 1e9, 1),
         "effective_GBps_survey_bytes": round(batched_mpx * 1e6 * SURVEY_FP16_FRAME_BYTES_PX / 1e9, 1),
         "stage_ms": None if stage_ms is None else {n: round(float(t), 5) for n, t in zip(names, stage_ms)},
