@@ -689,7 +689,8 @@ def main():
 
     if args.batched_only:
         print(json.dumps({"batched_fp16_mpx_per_s": round(batched_mpx, 2), "launch": head, "steps": args.steps,
-                          "ms_per_step": round(t_batched / args.steps * 1e3, 4), "pixels_per_batch": batch_px}))
+                          "ms_per_step": round(t_batched / args.steps * 1e3, 4), "pixels_per_batch": batch_px,
+                          "stage_ms": None if stage_ms is None else [round(float(t), 5) for t in stage_ms]}))
         return
 
     progress("plain FP32 canonical form on the same mix")

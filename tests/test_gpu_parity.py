@@ -152,6 +152,40 @@ def test_canonical_fp32_sequence(S, oracle_lib):
         assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), "history length differs"
 
 
+@pytest.mark.parametrize("form", ["fp32", "fp16_wide"])
+def test_canonical_incoherent_motion(S, oracle_lib, form):
+    """The temporal stage reads a pixel's four history taps from the tile's staged footprint
+    (the tile moved by its centre pixel's motion) when they lie in it, and gathers them
+    otherwise. Motion fields that mix both within tiles: per-pixel jitter of up to +-2.5 px, bands
+    of a far coherent shift (-7.3 px), and pixels with the no-motion sentinel (-1e6) or motion
+    off the frame. 4 frames against the FP64 oracle, history lengths exact."""
+    O = oracle_lib
+    w, h = 150, 90
+    fp16 = form != "fp32"
+    inputs = seq_inputs(w, h, 4, noise=1 if fp16 else 0, pan_px=1.37, disocclusion=0.1, half=fp16)
+    rng = np.random.default_rng(17)
+    for f, d in enumerate(inputs):
+        if f == 0:
+            continue
+        m = d["motion"].copy()
+        m += rng.uniform(-2.5, 2.5, m.shape).astype(np.float32)
+        m[20:40, :, 0] = -7.3
+        m[:, 100:120, 1] = 5.6
+        m[rng.random((h, w)) < 0.02] = -1e6
+        m[60:70, 140:] = (40.0, 0.0)  # off the frame
+        d["motion"] = m.astype(np.float32)
+    op = O.canonical_params(5, compensated=fp16)
+    ref = oracle_canonical_sequence(inputs, op)
+    storage, dt = (S.STORAGE_FP16, torch.float16) if fp16 else (S.STORAGE_FP32, torch.float32)
+    with S.Context(S.MODE_CANONICAL, storage, max_width=w, max_height=h) as ctx:
+        got = gpu_canonical_sequence(ctx, inputs, dt)
+    for f, (g, r) in enumerate(zip(got, ref)):
+        dm, dmean = rel_err(g["denoised"][..., :3], r["denoised"][..., :3])
+        print(f"{form} frame {f}: denoised max {dm:.3e} mean {dmean:.3e}")
+        assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), f
+        assert dm <= (1e-3 if fp16 else 2e-4), f
+
+
 @pytest.mark.parametrize("phis", [(4.0, 0.0, 0.0), (4.0, 8.0, 0.5), (2.0, 128.0, 0.0), (10.0, 0.0, 3.0)],
                          ids=["luma_only", "weak_normal", "no_depth", "no_normal"])
 def test_canonical_edge_stop_settings(S, oracle_lib, phis):
