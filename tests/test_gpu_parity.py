@@ -484,3 +484,35 @@ def test_single_frame_calls_on_two_streams(S, mode):
             for k in ("denoised", "moments_curr", "history_curr"):
                 if a[k] is not None:
                     assert torch.equal(a[k].view(torch.int16), b[k].view(torch.int16)), k
+
+
+def test_set_params_keeps_the_state_form_of_open_streams(S):
+    """The moment / history form (compensated, wide) cannot change while resident streams hold
+    state in the old one; other parameters can. With no stream open the form changes, and the
+    moments planes follow it."""
+    dev = torch.device("cuda")
+    w, h = 64, 32
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=w, max_height=h, max_streams=2, max_batch_frames=2,
+                   max_batch_pixels=2 * w * h) as ctx:
+        assert ctx.wide and ctx.moment_channels == 4
+        sid = ctx.stream_open(w, h)
+        plain = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
+        plain.flags = S.FLAG_COMPENSATED
+        with pytest.raises(S.SvgfError) as e:
+            ctx.set_params(plain)
+        assert e.value.status == 1 and ctx.wide
+        q = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
+        q.phi_luma = 6.0
+        ctx.set_params(q)
+        ctx.stream_close(sid)
+        ctx.set_params(plain)
+        assert not ctx.wide and ctx.moment_channels == 2
+        d = S.synth_frame(w, h, seed=2, frame=0)
+        sid = ctx.stream_open(w, h)
+        den = torch.empty((h, w, 4), dtype=torch.float16, device=dev)
+        ctx.denoise_batch([{"stream_id": sid, "radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                            "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                            "motion": torch.from_numpy(d["motion"]).to(dev), "denoised": den}])
+        assert ctx.batch_status(ctx.batch_last()) == 0
+        st = ctx.stream_state(sid)
+        assert st.moments.format == S.FMT_RG16F
