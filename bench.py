@@ -38,17 +38,19 @@ def progress(msg: str) -> None:
 # Algorithmic bytes per pixel, per stage, stage-structured rule: each stage reads each distinct
 # plane once and writes each output plane once at its storage width (extra taps count 0).
 #   temporal : radiance + G + motion + G_prev + history + moments read; wavelet W0 (colour,
-#              sigma; FP32) + moments + length (u8) + depth gradient (2 x FP32) written (+ the
-#              G copy kept for the next frame in the batched form)
+#              sigma; FP32) + moments + length (u8) + depth gradient (2 x FP32) written (+ a
+#              copy of G for the next frame when the batched form saves one: requests without
+#              SVGF_REQUEST_KEEP_GBUFFER)
 #   a-trous  : wavelet (FP32) + G + depth gradient read; wavelet written (the last iteration
 #              writes the output instead); iteration 0 also reads the length and writes the
 #              colour history, and computes the spatial variance of young pixels (n < 4) from
 #              the region it already holds in LDS (no extra plane); at stride 1 it also takes
 #              the depth gradient from the staged depths instead of the gradient plane
-def stage_bytes(storage_bytes4, storage_bytes2, iterations, batched):
-    """storage_bytes2: bytes of a moments pixel (8 for the wide FP16 state's four binary16 words)."""
+def stage_bytes(storage_bytes4, storage_bytes2, iterations, g_copy):
+    """storage_bytes2: bytes of a moments pixel (8 for the wide FP16 state's four binary16 words);
+    g_copy: the temporal stage writes a copy of G for the stream's next frame."""
     s4, s2, w4 = storage_bytes4, storage_bytes2, 16
-    temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (w4 + s2 + 1 + 8) + (16 if batched else 0)
+    temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (w4 + s2 + 1 + 8) + (16 if g_copy else 0)
     atrous = [(w4 + 16) + (0 if k == 0 else 8) + (s4 if k == iterations - 1 else w4)
               + ((1 + s4) if k == 0 else 0) for k in range(iterations)]
     return [temporal] + atrous
@@ -491,6 +493,10 @@ def main():
     ap.add_argument("--sequence", choices=["pingpong", "cut"], default="pingpong",
                     help="how the distinct frames repeat: forward and back (continuous), or in a loop (history "
                          "rejected at each restart)")
+    ap.add_argument("--gbuffer", choices=["kept", "copied"], default="kept",
+                    help="batched requests: the caller keeps each frame's G-buffer for the stream's next frame "
+                         "(SVGF_REQUEST_KEEP_GBUFFER; the input frames never change), or the temporal stage saves "
+                         "a copy")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP64-oracle check of the timed batched output")
@@ -608,6 +614,7 @@ def main():
         for i, (w, h) in enumerate(shapes):
             r = arr[i]
             r.stream_id = i  # every batched context below opens its streams in this order
+            r.flags = S.REQUEST_KEEP_GBUFFER if args.gbuffer == "kept" else 0
             r.radiance = S.plane_of(rad16[i][f], S.FMT_RGBA16F)
             r.gbuffer = S.plane_of(gbuf[i][f], S.FMT_RGBA32F)
             r.motion = S.plane_of(mvec[i][f] if sgn > 0 else mneg[i][f], S.FMT_RG32F)
@@ -735,7 +742,7 @@ def main():
     plain_mpx = args.steps * batch_px / t_plain / 1e6
 
     names = ["temporal"] + [f"atrous{k}" for k in range(p16.iterations)]
-    sb16 = stage_bytes(8, 8 if p16.flags & S.FLAG_WIDE_STATE else 4, p16.iterations, batched=True)
+    sb16 = stage_bytes(8, 8 if p16.flags & S.FLAG_WIDE_STATE else 4, p16.iterations, g_copy=args.gbuffer == "copied")
     # Dominant kernel (largest share of the batch's GPU time): the residue-pair a-trous kernel of
     # the iterations before the last (strides 2..2^(N-2): one code object, N-2 launches per
     # batch; the last iteration is another kernel, which skips the filtered variance). `achieved`
@@ -748,7 +755,7 @@ def main():
     achieved = SURVEY_FP16_ATROUS_BYTES_PX * batch_px / (dom_ms * 1e-3) / 1e9
     layout_achieved = layout_bytes_px * batch_px / (dom_ms * 1e-3) / 1e9
     bytes_px_16 = sum(sb16)
-    bytes_px_32 = sum(stage_bytes(16, 8, p32.iterations, batched=False))
+    bytes_px_32 = sum(stage_bytes(16, 8, p32.iterations, g_copy=False))
 
     progress("the reference's own published case")
     # ---- the reference's own published case: REFERENCE mode, rgba16f, 1280x720, init + 3 passes ----
@@ -947,7 +954,11 @@ def main():
             "plane_layout": "radiance rgba16f; history rgba16f (alpha: length + 3-bit rounding residual per "
                             "channel); moments rgba16f (mean, sigma as binary16 hi + scaled lo words); wavelet "
                             "(colour, sigma) rgba32f; G normal.xyz+depth rgba32f; motion rg32f; depth gradient "
-                            "rg32f; history length u8; G copy rgba32f",
+                            "rg32f; history length u8" + ("; G copy rgba32f" if args.gbuffer == "copied" else ""),
+            "gbuffer": ("kept: each request carries SVGF_REQUEST_KEEP_GBUFFER, so the stream's next frame "
+                        "reads that G-buffer in place as its previous one (the single-frame form likewise reads "
+                        "the caller's previous G-buffer)" if args.gbuffer == "kept" else
+                        "copied: the temporal stage saves a copy of G in the stream's state for the next frame"),
             "survey_bytes_per_px": SURVEY_FP16_FRAME_BYTES_PX,
         },
         "accuracy": accuracy,

@@ -167,10 +167,19 @@ typedef struct svgf_frame {
 } svgf_frame;
 
 /* One request of the batched form: per-frame inputs and output; history is library-resident
- * per stream (opened with svgf_stream_open). */
+ * per stream (opened with svgf_stream_open). The stream's next frame reprojects against this
+ * frame's G-buffer: by default the temporal stage saves a copy of it in the stream's state
+ * (16 B per pixel written). */
+/* svgf_request.flags (CANONICAL; other bits are SVGF_ERR_INVALID_ARGUMENT):
+ * the caller leaves this request's gbuffer plane unchanged and allocated until the stream's
+ * next frame has completed on the GPU (a G-buffer kept per frame in flight, as the renderer
+ * keeps its per-frame resources by main.cpp:3148's current_frame flip, and written on the
+ * stream that runs the batches meets this). The next frame then reads it in place as its
+ * previous G-buffer, and no copy is saved. Results are bit-identical to the copying form. */
+#define SVGF_REQUEST_KEEP_GBUFFER 0x1u
 typedef struct svgf_request {
     uint32_t stream_id;
-    uint32_t reserved;
+    uint32_t flags; /* SVGF_REQUEST_* */
     svgf_plane radiance;
     svgf_plane gbuffer; /* CANONICAL only */
     svgf_plane motion;  /* CANONICAL only */
@@ -253,7 +262,7 @@ SVGF_API svgf_status svgf_batch_last(svgf_context* ctx, uint64_t* batch);
 /* Outcome of a batch: SVGF_OK when it completed correctly, SVGF_ERR_TIMEOUT when a dependency
  * wait of its persistent launch gave up (its outputs are wrong; its streams have been reset to
  * cold history), SVGF_ERR_NOT_READY while it runs (wait = 0; wait = 1 blocks until it has
- * completed). The outcomes of the last 3 batches are kept; an older or unknown number is
+ * completed). The outcomes of the last 64 batches are kept; an older or unknown number is
  * SVGF_ERR_INVALID_ARGUMENT. */
 SVGF_API svgf_status svgf_batch_status(svgf_context* ctx, uint64_t batch, int32_t wait);
 

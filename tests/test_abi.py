@@ -217,6 +217,7 @@ def test_binding_flags_match_header():
     assert defs["FLAG_COMPENSATED_MOMENTS"] == S.FLAG_COMPENSATED
     assert defs["FLAG_WIDE_STATE"] == S.FLAG_WIDE_STATE
     assert defs["BATCH_TIMED_STAGES"] == S.BATCH_TIMED_STAGES
+    assert defs["REQUEST_KEEP_GBUFFER"] == S.REQUEST_KEEP_GBUFFER
 
 
 def test_wire_and_readback_argument_checks_without_gpu():

@@ -516,3 +516,31 @@ def test_set_params_keeps_the_state_form_of_open_streams(S):
         assert ctx.batch_status(ctx.batch_last()) == 0
         st = ctx.stream_state(sid)
         assert st.moments.format == S.FMT_RG16F
+
+
+def test_batch_outcomes_outlive_their_slots(S):
+    """A batch's outcome stays readable after later batches have reused its slot (a serving
+    loop's lane may read it only after the other lanes have cycled): ten batches in a row on one
+    stream, then every outcome, oldest first; an outcome 64 batches old is gone."""
+    dev = torch.device("cuda")
+    w, h = 64, 48
+    d = S.synth_frame(w, h, seed=9, frame=0, pan_px=0.0)
+    item = {"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+            "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+            "motion": torch.from_numpy(d["motion"]).to(dev),
+            "denoised": torch.empty((h, w, 4), dtype=torch.float16, device=dev)}
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=w, max_height=h, max_streams=1,
+                   max_batch_frames=1, max_batch_pixels=w * h) as ctx:
+        sid = ctx.stream_open(w, h)
+        first = None
+        for _ in range(10):
+            ctx.denoise_batch([{"stream_id": sid, **item}])
+            first = first or ctx.batch_last()
+        torch.cuda.synchronize()
+        for b in range(first, first + 10):
+            assert ctx.batch_status(b, wait=True) == 0, b
+        for _ in range(64):
+            ctx.denoise_batch([{"stream_id": sid, **item}])
+        torch.cuda.synchronize()
+        assert ctx.batch_status(first, wait=True) == 1  # evicted: invalid argument
+        assert ctx.batch_status(ctx.batch_last(), wait=True) == 0

@@ -21,6 +21,10 @@ using namespace svgf;
 namespace {
 
 constexpr int BATCH_SLOTS = 3;
+// Outcomes of completed batches kept for svgf_batch_status after their slot is reused (a
+// caller with several batches in flight, such as the serving loop's lanes, may read a batch's
+// outcome only after later batches have cycled through the slots).
+constexpr int BATCH_OUTCOMES = 64;
 constexpr size_t ALIGN = 256;
 
 constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -44,6 +48,10 @@ struct StreamSlot {
     void* block = nullptr;
     uint64_t bytes = 0;
     PlaneDev hist[2]{}, mom[2]{}, gbuf[2]{};
+    // The last frame was requested with SVGF_REQUEST_KEEP_GBUFFER: the next one reads that
+    // request's G-buffer (gkept) instead of gbuf[parity].
+    bool g_kept = false;
+    PlaneDev gkept{};
 };
 
 struct BatchSlot {
@@ -92,6 +100,10 @@ struct svgf_context {
     BatchSlot slots[BATCH_SLOTS];
     int next_slot = 0;
     uint64_t last_seq = 0;  // id of the last batch enqueued
+    struct Outcome {
+        uint64_t seq = 0;
+        bool failed = false;
+    } outcomes[BATCH_OUTCOMES];  // [seq % BATCH_OUTCOMES]: checked batches
     size_t table_bytes = 0, starts_off = 0, frames_off = 0, arena_bytes = 0;
     int grid_blocks = 0;
     uint32_t persist_blocks = 0;  // resident workgroups of the persistent kernel on the device
@@ -248,6 +260,7 @@ void check_slot(svgf_context* c, BatchSlot& s)
 {
     if (s.pending || s.checked || !s.seq) return;
     s.checked = true;
+    auto record = [&] { c->outcomes[s.seq % BATCH_OUTCOMES] = {s.seq, s.failed}; };
     if (s.sev_n > 0) {
         // stage times of a batch recorded with SVGF_BATCH_TIMED_STAGES
         float t[MAX_STAGES];
@@ -263,11 +276,12 @@ void check_slot(svgf_context* c, BatchSlot& s)
             c->stage_batches++;
         }
     }
-    if (!s.err_host || *s.err_host == 0) return;
+    if (!s.err_host || *s.err_host == 0) return record();
     *s.err_host = 0;
     s.failed = true;
     for (uint32_t id : s.ids)
         if (id < c->streams.size() && c->streams[id].open) c->streams[id].frames = 0;
+    record();
 }
 
 // Marks completed slots (non-blocking) and reads their outcome.
@@ -919,6 +933,7 @@ svgf_status svgf_stream_reset(svgf_context* c, uint32_t id, void* hip_stream)
     // Cold history: the next frame runs with history_valid = 0 and never reads the old state
     // (the zero-cleared images of main.cpp:1449-1455 / 1570-1603).
     s->frames = 0;
+    s->g_kept = false;
     return SVGF_OK;
 }
 
@@ -935,7 +950,8 @@ svgf_status svgf_stream_state_get(svgf_context* c, uint32_t id, svgf_stream_stat
     out->moments = svgf_plane{s->mom[p].ptr, s->mom[p].pitch, fmt2(c), 0};
     if (c->desc.mode == SVGF_MODE_CANONICAL) {
         out->history = svgf_plane{s->hist[p].ptr, s->hist[p].pitch, fmt4(c), 0};
-        out->gbuffer = svgf_plane{s->gbuf[p].ptr, s->gbuf[p].pitch, SVGF_FMT_RGBA32F, 0};
+        const PlaneDev& g = s->g_kept ? s->gkept : s->gbuf[p];
+        out->gbuffer = svgf_plane{g.ptr, g.pitch, SVGF_FMT_RGBA32F, 0};
     }
     return SVGF_OK;
 }
@@ -959,6 +975,7 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
         if (!s) return SVGF_ERR_NO_STREAM;
         for (uint32_t j = 0; j < i; ++j)
             if (req[j].stream_id == req[i].stream_id) return SVGF_ERR_INVALID_ARGUMENT;
+        if (req[i].flags & ~SVGF_REQUEST_KEEP_GBUFFER) return SVGF_ERR_INVALID_ARGUMENT;
         svgf_status st;
         if ((st = check_plane(req[i].radiance, fmt4(c), s->w)) != SVGF_OK) return st;
         if ((st = check_plane(req[i].denoised, fmt4(c), s->w)) != SVGF_OK) return st;
@@ -1015,8 +1032,9 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
         if (c->mode == SVGF_MODE_CANONICAL) {
             f.g = dev_plane(req[i].gbuffer);
             if (f.history_valid) f.mv = dev_plane(req[i].motion);
-            f.gprev = s.gbuf[rd];
-            f.gsave = s.gbuf[wr];
+            f.gprev = s.g_kept ? s.gkept : s.gbuf[rd];
+            // kept by the caller: the next frame reads it in place
+            if (!(req[i].flags & SVGF_REQUEST_KEEP_GBUFFER)) f.gsave = s.gbuf[wr];
             f.hist_prev = s.hist[rd];
             f.hist_curr = s.hist[wr];
         }
@@ -1064,6 +1082,8 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
         StreamSlot& s = c->streams[req[i].stream_id];
         s.frames++;
         s.parity ^= 1;
+        s.g_kept = c->mode == SVGF_MODE_CANONICAL && (req[i].flags & SVGF_REQUEST_KEEP_GBUFFER);
+        s.gkept = s.g_kept ? frames[i].g : PlaneDev{};
     }
     return SVGF_OK;
 }
@@ -1081,7 +1101,12 @@ svgf_status svgf_batch_status(svgf_context* c, uint64_t batch, int32_t wait)
     BatchSlot* s = nullptr;
     for (auto& sl : c->slots)
         if (sl.seq == batch) s = &sl;
-    if (!s) return SVGF_ERR_INVALID_ARGUMENT;  // never enqueued, or its slot was reused since
+    if (!s) {
+        // its slot was reused since (the reuse waited for it and recorded its outcome)
+        const auto& o = c->outcomes[batch % BATCH_OUTCOMES];
+        if (o.seq != batch) return SVGF_ERR_INVALID_ARGUMENT;  // never enqueued, or too old
+        return o.failed ? SVGF_ERR_TIMEOUT : SVGF_OK;
+    }
     DeviceGuard guard(c->desc.device);
     if (s->pending) {
         const hipError_t q = wait ? hipEventSynchronize(s->done) : hipEventQuery(s->done);

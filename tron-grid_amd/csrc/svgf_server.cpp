@@ -254,7 +254,7 @@ svgf_status svgf_server_stream_close(svgf_server* s, uint32_t id)
 
 svgf_status svgf_server_submit(svgf_server* s, const svgf_request* req, uint64_t tag)
 {
-    if (!s || !req) return SVGF_ERR_INVALID_ARGUMENT;
+    if (!s || !req || (req->flags & ~SVGF_REQUEST_KEEP_GBUFFER)) return SVGF_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return SVGF_ERR_INVALID_ARGUMENT;
     svgf_stream_state st{};

@@ -34,6 +34,7 @@ BATCH_NO_GRAPH = 0x1
 PROFILE_STAGES = 0x2
 BATCH_PERSISTENT = 0x4
 BATCH_TIMED_STAGES = 0x8
+REQUEST_KEEP_GBUFFER = 0x1
 
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "unsupported", 3: "out of device memory", 4: "over HBM budget",
@@ -97,7 +98,7 @@ class Frame(ctypes.Structure):
 
 
 class Request(ctypes.Structure):
-    _fields_ = [("stream_id", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("radiance", Plane),
+    _fields_ = [("stream_id", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("radiance", Plane),
                 ("gbuffer", Plane), ("motion", Plane), ("denoised", Plane)]
 
 
@@ -483,12 +484,14 @@ class Context:
         return [buf[i] for i in range(n.value)]
 
     def denoise_batch(self, items, stream=None):
-        """items: list of dicts {stream_id, radiance, denoised, gbuffer?, motion?} (tensors)."""
+        """items: list of dicts {stream_id, radiance, denoised, gbuffer?, motion?} (tensors) and
+        optional request flags (REQUEST_KEEP_GBUFFER)."""
         st = self.storage
         arr = (Request * max(1, len(items)))()
         for i, it in enumerate(items):
             r = arr[i]
             r.stream_id = it["stream_id"]
+            r.flags = it.get("flags", 0)
             r.radiance = plane_of(it["radiance"], fmt4(st))
             r.denoised = plane_of(it["denoised"], fmt4(st))
             r.gbuffer = plane_of(it.get("gbuffer"), FMT_RGBA32F)
@@ -538,6 +541,7 @@ class Server:
         st = self.ctx.storage
         r = Request()
         r.stream_id = item["stream_id"]
+        r.flags = item.get("flags", 0)
         r.radiance = plane_of(item["radiance"], fmt4(st))
         r.denoised = plane_of(item["denoised"], fmt4(st))
         r.gbuffer = plane_of(item.get("gbuffer"), FMT_RGBA32F)
@@ -563,6 +567,7 @@ class Server:
             assert len(ins) == ipc
             for k, it in enumerate(ins):
                 r = arr[c * ipc + k]
+                r.flags = it.get("flags", 0)
                 r.radiance = plane_of(it["radiance"], fmt4(st))
                 r.gbuffer = plane_of(it.get("gbuffer"), FMT_RGBA32F)
                 r.motion = plane_of(it.get("motion"), FMT_RG32F)
