@@ -877,14 +877,19 @@ def main():
                                        "frames": reps}
 
     traffic = args.traffic
-    if traffic is None:
-        # HBM bytes per a-trous launch, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-        # workload and recorded beside the other profile summaries
-        tfile = ROOT / "profiles" / "traffic.txt"
-        if tfile.exists() and args.frames == 64:
-            for ln in tfile.read_text().splitlines():
-                if ln.startswith("atrous_pair_wide_batched_fp16_mix64_bytes_per_launch:"):
-                    traffic = float(ln.split(":", 1)[1])
+    valu_insts = None
+    tfile = ROOT / "profiles" / "traffic.txt"
+    if tfile.exists() and args.frames == 64:
+        # HBM bytes and VALU instructions per a-trous launch, measured with rocprofv3 (FETCH_SIZE /
+        # WRITE_SIZE, SQ_INSTS_VALU) on this workload and recorded beside the profile summaries
+        for ln in tfile.read_text().splitlines():
+            if ln.startswith("atrous_pair_wide_batched_fp16_mix64_bytes_per_launch:") and traffic is None:
+                traffic = float(ln.split(":", 1)[1])
+            if ln.startswith("atrous_pair_wide_batched_fp16_mix64_valu_insts_per_launch:"):
+                valu_insts = float(ln.split(":", 1)[1])
+    # VALU issue rate of the chip: one wave64 instruction per SIMD every 4 cycles (16 lanes per
+    # cycle), 4 SIMDs per CU, 256 CUs, at the 2.4 GHz peak engine clock
+    valu_issue_per_s = 256 * 4 * 2.4e9 / 4
 
     progress("accuracy of the timed batched output")
     accuracy = None
@@ -1021,7 +1026,14 @@ def main():
                                       "graphs of the timed batches (SVGF_BATCH_TIMED_STAGES), on the launch "
                                       "stream; mean over the kernel's launches in the timed region",
                      "note": "arithmetic-bound: ~16 VALU instructions (2 transcendental) per tap, 25 taps per "
-                             "pixel, with the VALU and LDS the busiest units; HBM fraction is not the binding limit"},
+                             "pixel, with the VALU and LDS the busiest units; HBM fraction is not the binding limit",
+                     "valu": None if valu_insts is None or not np.isfinite(dom_ms) else {
+                         "instructions_per_launch": int(valu_insts),
+                         "issue_frac": round(valu_insts / (dom_ms * 1e-3) / valu_issue_per_s, 4),
+                         "peak": "one wave64 VALU instruction per SIMD per 4 cycles, 1024 SIMDs, 2.4 GHz "
+                                 "(a lower clock under load raises the fraction)",
+                         "source": "profiles/traffic.txt: rocprofv3 SQ_INSTS_VALU per launch of this kernel on "
+                                   "this workload (committed figure) over this run's mean launch time"}},
         "cpu_baseline": cpu,
         "input_generation_s": round(gen_s, 2),
     }
