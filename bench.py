@@ -681,7 +681,7 @@ def main():
     head = args.launch
     t_batched, lat, stage_ms, sampled = time_batched(forms[head] | S.BATCH_TIMED_STAGES, capture)
     batched_mpx = args.steps * batch_px / t_batched / 1e6
-    other = plain_graph = None
+    other = plain_graph = copied = None
     if not args.batched_only:
         name = "per_stage" if head == "persistent" else "persistent"
         t_o, lat_o, _, _ = time_batched(forms[name])
@@ -693,6 +693,21 @@ def main():
         plain_graph = {"value": round(args.steps * batch_px / t_n / 1e6, 2), "unit": "Mpx/s",
                        "ms_per_step": round(t_n / args.steps * 1e3, 4),
                        "p99_ms": round(float(np.percentile(lat_n, 99)), 4)}
+        if args.gbuffer == "kept":
+            # the same form with requests that do not keep their G-buffer: the temporal stage
+            # saves a copy of G for each stream's next frame (same bits)
+            for arr in reqs:
+                for r in arr:
+                    r.flags = 0
+            t_c, lat_c, _, _ = time_batched(forms[head] | S.BATCH_TIMED_STAGES)
+            for arr in reqs:
+                for r in arr:
+                    r.flags = S.REQUEST_KEEP_GBUFFER
+            copied = {"value": round(args.steps * batch_px / t_c / 1e6, 2), "unit": "Mpx/s",
+                      "ms_per_step": round(t_c / args.steps * 1e3, 4),
+                      "p99_ms": round(float(np.percentile(lat_c, 99)), 4),
+                      "what": "the headline form with the temporal stage saving a copy of each G-buffer "
+                              "(requests without SVGF_REQUEST_KEEP_GBUFFER)"}
 
     if args.batched_only:
         print(json.dumps({"batched_fp16_mpx_per_s": round(batched_mpx, 2), "launch": head, "steps": args.steps,
@@ -984,6 +999,7 @@ def main():
         "producer_pack": pack,
         "batched_other_launch": other,
         "batched_without_stage_events": plain_graph,
+        "batched_gbuffer_copied": copied,
         "plain_fp32": {"value": round(plain_mpx, 2), "unit": "Mpx/s", "bytes_per_px": bytes_px_32,
                        "ratio_batched_over_plain": round(batched_mpx / plain_mpx, 3)},
         # the stated targets of the batched FP16 form and where this run stands

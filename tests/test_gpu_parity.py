@@ -153,6 +153,31 @@ def test_canonical_fp32_sequence(S, oracle_lib):
 
 
 @pytest.mark.parametrize("form", ["fp32", "fp16_wide"])
+@pytest.mark.parametrize("shape", [(1, 1), (2, 9), (9, 2), (17, 3), (65, 5), (33, 257), (67, 131)])
+def test_canonical_edge_sizes(S, oracle_lib, shape, form):
+    """Frames narrower or shorter than the a-trous strides (lattice residues with no pixel,
+    single-point lattices, tiles cut by both edges), 4 frames each, vs the FP64 oracle: FP32
+    within 2e-4, the FP16 wide form within 1e-3 max relative error."""
+    O = oracle_lib
+    w, h = shape
+    fp16 = form == "fp16_wide"
+    inputs = seq_inputs(w, h, 4, seed0=50, pan_px=0.9, disocclusion=0.1, half=fp16)
+    ref = oracle_canonical_sequence(inputs, O.canonical_params(5, compensated=fp16))
+    storage, dt = (S.STORAGE_FP16, torch.float16) if fp16 else (S.STORAGE_FP32, torch.float32)
+    bound = 1e-3 if fp16 else 2e-4
+    with S.Context(S.MODE_CANONICAL, storage, max_width=w, max_height=h) as ctx:
+        got = gpu_canonical_sequence(ctx, inputs, dt)
+    for f, (g, r) in enumerate(zip(got, ref)):
+        assert np.isfinite(g["denoised"]).all(), f
+        assert rel_err(g["denoised"][..., :3], r["denoised"][..., :3])[0] <= bound, f
+        assert rel_err(g["history"][..., :3], r["history"][..., :3])[0] <= bound, f
+        assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), f
+        if fp16:
+            me = moment_errors(g["moments"], r["moments"])
+            assert me["mean"][0] <= bound and me["second"][0] <= bound, f
+
+
+@pytest.mark.parametrize("form", ["fp32", "fp16_wide"])
 def test_canonical_incoherent_motion(S, oracle_lib, form):
     """The temporal stage reads a pixel's four history taps from the tile's staged footprint
     (the tile moved by its centre pixel's motion) when they lie in it, and gathers them
