@@ -863,22 +863,32 @@ def main():
 
         threads = max(1, min(16, os.cpu_count() or 1))
         op = O.canonical_params(p16.iterations, compensated=True)
-        done_px, t_cpu, frames_done = 0, 0.0, 0
-        for i, (w, h) in enumerate(shapes):
-            d0, d1 = host[i][0], host[i][1]
-            a = O.c_frame_f64(d0["radiance"], d0["gbuffer"], None, d0["motion"], None, None, op, history_valid=False,
-                              threads=threads)
-            t0 = time.perf_counter()
-            O.c_frame_f64(d1["radiance"], d1["gbuffer"], d0["gbuffer"], d1["motion"], a["history"], a["moments"],
-                          op, history_valid=True, threads=threads)
-            t_cpu += time.perf_counter() - t0
-            done_px += w * h
-            frames_done += 1
-            if t_cpu >= args.cpu_seconds:
-                break
-        cpu = {"value": round(done_px / t_cpu / 1e6, 4), "unit": "Mpx/s", "cores": threads, "kind": "port",
-               "sample": f"{frames_done} warm-history frames of the 64-frame mix ({done_px} px), CANONICAL "
-                         f"FP64 oracle (C, rows split over {threads} threads)"}
+
+        def cpu_rate(frame_fn, seconds):
+            # warm-history frames of the 64-frame mix, in stream order, until `seconds` of work
+            done_px, t_cpu, frames_done = 0, 0.0, 0
+            for i, (w, h) in enumerate(shapes):
+                d0, d1 = host[i][0], host[i][1]
+                a = frame_fn(d0["radiance"], d0["gbuffer"], None, d0["motion"], None, None, op, history_valid=False,
+                             threads=threads)
+                t0 = time.perf_counter()
+                frame_fn(d1["radiance"], d1["gbuffer"], d0["gbuffer"], d1["motion"], a["history"], a["moments"],
+                         op, history_valid=True, threads=threads)
+                t_cpu += time.perf_counter() - t0
+                done_px += w * h
+                frames_done += 1
+                if t_cpu >= seconds:
+                    break
+            return done_px / t_cpu / 1e6, frames_done, done_px
+
+        rate, frames_done, done_px = cpu_rate(O.c_frame_f32, args.cpu_seconds)
+        rate64, frames64, px64 = cpu_rate(O.c_frame_f64, args.cpu_seconds / 2)
+        cpu = {"value": round(rate, 4), "unit": "Mpx/s", "cores": threads, "kind": "port",
+               "sample": f"{frames_done} warm-history frames of the 64-frame mix ({done_px} px), CANONICAL in "
+                         f"binary32 (the oracle's C code built for float, compensated moments; rows split over "
+                         f"{threads} threads)",
+               "oracle_fp64": {"mpx_per_s": round(rate64, 4), "frames": frames64, "pixels": px64,
+                               "what": "the same frames through the binary64 oracle"}}
         # the reference's own semantics on its CPU path: REFERENCE-mode float restatement, 1080p
         L1080 = S.synth_frame(1920, 1080, seed=0, frame=1, pan_px=0.5)["radiance"]
         M1080 = O.r_frame_f32(L1080, None, O.reference_params(3), history_valid=False, threads=threads)[0]

@@ -77,6 +77,8 @@ def lib():
         L.oracle_c_frame_f64.argtypes = [ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, fp, fp, fp, fp,
                                          dp, dp, dp, dp, dp, dp, dp]
         L.oracle_c_frame_f64_mt.argtypes = L.oracle_c_frame_f64.argtypes + [ctypes.c_int]
+        L.oracle_c_frame_f32_mt.argtypes = [ctypes.c_int, ctypes.c_int, pp, ctypes.c_int, fp, fp, fp, fp,
+                                            fp, fp, fp, fp, fp, fp, fp, ctypes.c_int]
         L.oracle_f32_to_f16.argtypes = [ctypes.c_float]
         L.oracle_f32_to_f16.restype = ctypes.c_uint16
         L.oracle_f16_to_f32.argtypes = [ctypes.c_uint16]
@@ -185,6 +187,27 @@ def c_frame_f64(radiance, gbuffer, gbuffer_prev, motion, history_prev, moments_p
         _dp(out["denoised"]), _dp(out.get("temporal")), _dp(out.get("variance")), int(threads))
     if rc:
         raise ValueError(f"oracle_c_frame_f64 failed: {rc}")
+    return out
+
+
+def c_frame_f32(radiance, gbuffer, gbuffer_prev, motion, history_prev, moments_prev, params=None,
+                history_valid=True, threads=1):
+    """One CANONICAL frame in float: the same code as c_frame_f64 in binary32 (the CPU baseline
+    port of the FP32 canonical form, not an oracle). Returns dict with history, moments,
+    denoised (float32)."""
+    p = params or canonical_params()
+    radiance = _f32(radiance)
+    h, w = radiance.shape[:2]
+    hist_prev = _f32(history_prev) if history_prev is not None else np.zeros((h, w, 4), np.float32)
+    mom_prev = _f32(moments_prev) if moments_prev is not None else np.zeros((h, w, 2), np.float32)
+    out = {k: np.zeros((h, w, c), np.float32) for k, c in (("history", 4), ("moments", 2), ("denoised", 4))}
+    gp = _f32(gbuffer_prev) if gbuffer_prev is not None else np.zeros((h, w, 4), np.float32)
+    rc = lib().oracle_c_frame_f32_mt(
+        w, h, ctypes.byref(p), int(bool(history_valid)), _fp(radiance), _fp(_f32(gbuffer)), _fp(gp),
+        _fp(_f32(motion)), _fp(hist_prev), _fp(mom_prev), _fp(out["history"]), _fp(out["moments"]),
+        _fp(out["denoised"]), None, None, int(threads))
+    if rc:
+        raise ValueError(f"oracle_c_frame_f32 failed: {rc}")
     return out
 
 

@@ -302,7 +302,8 @@ int oracle_r_frame_f64(int w, int h, const oracle_params* p, int history_valid,
 }
 
 /* ---------------------------------------------------------------------------------------
- * CANONICAL mode, binary64.
+ * CANONICAL mode: the oracle in binary64 (oracle_c_frame_f64*), and the same code in binary32
+ * (oracle_c_frame_f32_mt) as the CPU baseline port of the FP32 canonical form.
  *
  * Temporal stage (per pixel p = (x, y)):
  *   l = luma(radiance); reprojected position = (x + floor(m.x) + fx, y + floor(m.y) + fy),
@@ -330,420 +331,40 @@ int oracle_r_frame_f64(int w, int h, const oracle_params* p, int history_valid,
  *   Iteration 0's colour (with n in alpha) is the next frame's history; the last iteration
  *   writes denoised = (colour, 1).
  * ------------------------------------------------------------------------------------- */
-static inline double luma_d(const double* c) { return c[0] * (double)LUMA_R + c[1] * (double)LUMA_G + c[2] * (double)LUMA_B; }
 
-static double depth_normal_weight(const oracle_params* p, const float* gp, const float* gq, double gx,
-                                  double gy, double ox, double oy, double* arg_out)
-{
-    double arg = 0.0;
-    if (p->phi_depth > 0.0f) {
-        double den = (double)p->phi_depth * fabs(gx * ox + gy * oy) + (double)p->depth_eps;
-        arg = fabs((double)gq[3] - (double)gp[3]) / den;
-    }
-    double wn = 1.0;
-    if (p->phi_normal > 0.0f) {
-        double d = (double)gp[0] * gq[0] + (double)gp[1] * gq[1] + (double)gp[2] * gq[2];
-        wn = d > 0.0 ? pow(d, (double)p->phi_normal) : 0.0;
-    }
-    *arg_out = arg;
-    return wn;
-}
+#define CREAL double
+#define CSUF(name) name##_f64
+#define C_FRAME oracle_c_frame_f64
+#define C_FRAME_MT oracle_c_frame_f64_mt
+#define C_LIT(x) ((double)(x))
+#define C_EXP exp
+#define C_POW pow
+#define C_SQRT sqrt
+#define C_FABS fabs
+#define C_FLOOR floor
+#define C_FMAX fmax
+#include "svgf_oracle_canon.inc"
+#undef CREAL
+#undef CSUF
+#undef C_FRAME
+#undef C_FRAME_MT
+#undef C_LIT
+#undef C_EXP
+#undef C_POW
+#undef C_SQRT
+#undef C_FABS
+#undef C_FLOOR
+#undef C_FMAX
 
-static void depth_gradient(const float* g, int w, int h, int x, int y, double* gx, double* gy)
-{
-    int xl = clampi(x - 1, 0, w - 1), xr = clampi(x + 1, 0, w - 1);
-    int yu = clampi(y - 1, 0, h - 1), yd = clampi(y + 1, 0, h - 1);
-    *gx = 0.5 * ((double)g[4 * ((size_t)y * w + xr) + 3] - (double)g[4 * ((size_t)y * w + xl) + 3]);
-    *gy = 0.5 * ((double)g[4 * ((size_t)yd * w + x) + 3] - (double)g[4 * ((size_t)yu * w + x) + 3]);
-}
-
-typedef struct c_job {
-    int w, h, history_valid, as_ref, comp, k;
-    const oracle_params* p;
-    const float *radiance, *gbuffer, *gbuffer_prev, *motion;
-    const double *history_prev, *moments_prev;
-    double *history_curr, *moments_curr, *denoised;
-    double *C, *WA, *WB;
-    const double* src;
-    double* dst;
-    int* N;
-    int y0, y1;
-} c_job;
-
-static void c_temporal_rows(const c_job* J)
-{
-    const int w = J->w, h = J->h, history_valid = J->history_valid, as_ref = J->as_ref, comp = J->comp;
-    const oracle_params* p = J->p;
-    const float *radiance = J->radiance, *gbuffer = J->gbuffer, *gbuffer_prev = J->gbuffer_prev, *motion = J->motion;
-    const double *history_prev = J->history_prev, *moments_prev = J->moments_prev;
-    double *moments_curr = J->moments_curr, *C = J->C;
-    int* N = J->N;
-    const double alpha_m = p->alpha_moments, alpha_c = p->alpha_color;
-    for (int y = J->y0; y < J->y1; ++y) {
-        for (int x = 0; x < w; ++x) {
-            size_t i = (size_t)y * w + x;
-            const float* R = radiance + 4 * i;
-            double Rd[3] = {R[0], R[1], R[2]};
-            double l = luma_d(Rd);
-            if (as_ref) {
-                /* same-pixel moments, colour = radiance (svgf_denoise.slang:176-191) */
-                double m1 = history_valid ? moments_prev[2 * i] : 0.0;
-                double m2 = history_valid ? moments_prev[2 * i + 1] : 0.0;
-                double mu1 = m1 + alpha_m * (l - m1), mu2 = m2 + alpha_m * (l * l - m2);
-                moments_curr[2 * i] = mu1;
-                moments_curr[2 * i + 1] = mu2;
-                C[4 * i + 0] = Rd[0];
-                C[4 * i + 1] = Rd[1];
-                C[4 * i + 2] = Rd[2];
-                C[4 * i + 3] = sqrt(fmax(mu2 - mu1 * mu1, 0.0) + (double)p->variance_eps); /* sigma */
-                N[i] = 1000000;
-                continue;
-            }
-            const float* G = gbuffer + 4 * i;
-            double z = G[3];
-            double sumb = 0.0, hc[3] = {0, 0, 0}, hm1 = 0.0, hm2 = 0.0;
-            double tb[4], tmu[4], tm2[4];
-            int used = 0, nmax = 0;
-            if (history_valid) {
-                double mx = motion[2 * i], my = motion[2 * i + 1];
-                double flx = floor(mx), fly = floor(my);
-                double fx = mx - flx, fy = my - fly;
-                int x0 = x + (int)flx, y0 = y + (int)fly;
-                for (int jj = 0; jj < 2; ++jj) {
-                    for (int ii = 0; ii < 2; ++ii) {
-                        double b = (ii ? fx : 1.0 - fx) * (jj ? fy : 1.0 - fy);
-                        int qx = x0 + ii, qy = y0 + jj;
-                        if (!(b > 0.0) || qx < 0 || qx >= w || qy < 0 || qy >= h) continue;
-                        size_t q = (size_t)qy * w + qx;
-                        const float* Gq = gbuffer_prev + 4 * q;
-                        double dz = fabs((double)Gq[3] - z);
-                        if (dz > (double)p->depth_reject * z) continue;
-                        double nd = (double)Gq[0] * G[0] + (double)Gq[1] * G[1] + (double)Gq[2] * G[2];
-                        if (nd < (double)p->normal_reject) continue;
-                        const double* Hq = history_prev + 4 * q;
-                        int nq = (int)Hq[3];
-                        if (nq < 1) continue;
-                        sumb += b;
-                        hc[0] += b * Hq[0];
-                        hc[1] += b * Hq[1];
-                        hc[2] += b * Hq[2];
-                        if (nq > nmax) nmax = nq;
-                        if (comp) {
-                            tb[used] = b;
-                            tmu[used] = moments_prev[2 * q];
-                            tm2[used] = moments_prev[2 * q + 1] * moments_prev[2 * q + 1];
-                        } else {
-                            hm1 += b * moments_prev[2 * q];
-                            hm2 += b * moments_prev[2 * q + 1];
-                        }
-                        used++;
-                    }
-                }
-            }
-            int n_new = 1;
-            double hmu = 0.0, hvar = 0.0;
-            if (sumb > 0.0) {
-                n_new = nmax + 1;
-                if (n_new > (int)p->history_cap) n_new = (int)p->history_cap;
-                hc[0] /= sumb;
-                hc[1] /= sumb;
-                hc[2] /= sumb;
-                if (comp) {
-                    double mu = 0.0;
-                    for (int t = 0; t < used; ++t) mu += tb[t] * tmu[t];
-                    mu /= sumb;
-                    double m2 = 0.0;
-                    for (int t = 0; t < used; ++t) m2 += tb[t] * (tm2[t] + (tmu[t] - mu) * (tmu[t] - mu));
-                    hmu = mu;
-                    hvar = m2 / sumb;
-                } else {
-                    hm1 /= sumb;
-                    hm2 /= sumb;
-                }
-            } else {
-                hc[0] = hc[1] = hc[2] = 0.0;
-                hm1 = hm2 = 0.0;
-            }
-            double ac = fmax(alpha_c, 1.0 / n_new), am = fmax(alpha_m, 1.0 / n_new);
-            double col[3];
-            for (int c = 0; c < 3; ++c) col[c] = hc[c] + ac * (Rd[c] - hc[c]);
-            double var;
-            if (comp) {
-                double d = l - hmu;
-                double mu = hmu + am * d;
-                double m2 = (1.0 - am) * (hvar + am * d * d);
-                moments_curr[2 * i] = mu;
-                moments_curr[2 * i + 1] = sqrt(m2);
-                var = m2;
-            } else {
-                double mu1 = hm1 + am * (l - hm1), mu2 = hm2 + am * (l * l - hm2);
-                moments_curr[2 * i] = mu1;
-                moments_curr[2 * i + 1] = mu2;
-                var = fmax(mu2 - mu1 * mu1, 0.0);
-            }
-            C[4 * i + 0] = col[0];
-            C[4 * i + 1] = col[1];
-            C[4 * i + 2] = col[2];
-            C[4 * i + 3] = var;
-            N[i] = n_new;
-        }
-    }
-}
-
-static void c_variance_rows(const c_job* J)
-{
-    const int w = J->w, h = J->h, as_ref = J->as_ref;
-    const oracle_params* p = J->p;
-    const float* gbuffer = J->gbuffer;
-    const double* C = J->C;
-    double* WA = J->WA;
-    const int* N = J->N;
-    for (int y = J->y0; y < J->y1; ++y) {
-        for (int x = 0; x < w; ++x) {
-            size_t i = (size_t)y * w + x;
-            double var = C[4 * i + 3];
-            if (!as_ref && N[i] < 4) {
-                const float* Gp = gbuffer + 4 * i;
-                double gx, gy;
-                depth_gradient(gbuffer, w, h, x, y, &gx, &gy);
-                double tw[9], tl[9], sw = 0.0, s1 = 0.0;
-                int t = 0;
-                for (int dy = -1; dy <= 1; ++dy) {
-                    for (int dx = -1; dx <= 1; ++dx, ++t) {
-                        int qx = clampi(x + dx, 0, w - 1), qy = clampi(y + dy, 0, h - 1);
-                        size_t q = (size_t)qy * w + qx;
-                        double arg;
-                        double wn = depth_normal_weight(p, Gp, gbuffer + 4 * q, gx, gy, dx, dy, &arg);
-                        tw[t] = exp(-arg) * wn;
-                        tl[t] = luma_d(C + 4 * q);
-                        sw += tw[t];
-                        s1 += tw[t] * tl[t];
-                    }
-                }
-                if (sw > 0.0) {
-                    double mean = s1 / sw, s2 = 0.0;
-                    for (t = 0; t < 9; ++t) s2 += tw[t] * (tl[t] - mean) * (tl[t] - mean);
-                    var = s2 / sw;
-                } else {
-                    var = 0.0;
-                }
-            }
-            WA[4 * i + 0] = C[4 * i + 0];
-            WA[4 * i + 1] = C[4 * i + 1];
-            WA[4 * i + 2] = C[4 * i + 2];
-            WA[4 * i + 3] = var;
-        }
-    }
-}
-
-static void c_atrous_rows(const c_job* J)
-{
-    const int w = J->w, h = J->h, as_ref = J->as_ref, k = J->k;
-    const oracle_params* p = J->p;
-    const float* gbuffer = J->gbuffer;
-    const double* src = J->src;
-    double *dst = J->dst, *history_curr = J->history_curr, *denoised = J->denoised;
-    const int* N = J->N;
-    const int s = 1 << k;
-    const int fin = k == (int)p->iterations - 1;
-    for (int y = J->y0; y < J->y1; ++y) {
-        for (int x = 0; x < w; ++x) {
-            size_t i = (size_t)y * w + x;
-            const double* c = src + 4 * i;
-            double lc = luma_d(c);
-            double sigma;
-            double gx = 0.0, gy = 0.0;
-            const float* Gp = NULL;
-            if (as_ref) {
-                sigma = fmax((double)p->phi_luma * c[3], (double)p->variance_eps);
-            } else {
-                double gv = 0.0;
-                static const double K3[3] = {0.25, 0.5, 0.25};
-                const int ps = (p->flags & ORACLE_FLAG_PREFILTER_UNIT) ? 1 : s;
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        int qx = clampi(x + dx * ps, 0, w - 1), qy = clampi(y + dy * ps, 0, h - 1);
-                        gv += K3[dx + 1] * K3[dy + 1] * src[4 * ((size_t)qy * w + qx) + 3];
-                    }
-                if (p->flags & ORACLE_FLAG_EPS_OUTSIDE)
-                    sigma = (double)p->phi_luma * sqrt(fmax(gv, 0.0)) + (double)p->variance_eps;
-                else
-                    sigma = (double)p->phi_luma * sqrt(fmax(gv, 0.0) + (double)p->variance_eps);
-                Gp = gbuffer + 4 * i;
-                depth_gradient(gbuffer, w, h, x, y, &gx, &gy);
-            }
-            double s0 = 0, s1 = 0, s2 = 0, sw = 0, sv = 0;
-            for (int dy = -2; dy <= 2; ++dy) {
-                for (int dx = -2; dx <= 2; ++dx) {
-                    int qx = clampi(x + dx * s, 0, w - 1), qy = clampi(y + dy * s, 0, h - 1);
-                    size_t q = (size_t)qy * w + qx;
-                    const double* v = src + 4 * q;
-                    double arg = fabs(luma_d(v) - lc) / sigma;
-                    double wn = 1.0;
-                    if (!as_ref) {
-                        double az;
-                        wn = depth_normal_weight(p, Gp, gbuffer + 4 * q, gx, gy, (double)dx * s, (double)dy * s, &az);
-                        arg += az;
-                    }
-                    double wt = (double)KERNEL5[dx + 2] * (double)KERNEL5[dy + 2] * exp(-arg) * wn;
-                    s0 += wt * v[0];
-                    s1 += wt * v[1];
-                    s2 += wt * v[2];
-                    sw += wt;
-                    sv += wt * wt * v[3];
-                }
-            }
-            double* o = dst + 4 * i;
-            if (sw > 1e-6) {
-                o[0] = s0 / sw;
-                o[1] = s1 / sw;
-                o[2] = s2 / sw;
-                o[3] = as_ref ? c[3] : sv / (sw * sw);
-            } else {
-                o[0] = c[0];
-                o[1] = c[1];
-                o[2] = c[2];
-                o[3] = c[3];
-            }
-            if (k == 0 && !as_ref) {
-                history_curr[4 * i + 0] = o[0];
-                history_curr[4 * i + 1] = o[1];
-                history_curr[4 * i + 2] = o[2];
-                history_curr[4 * i + 3] = (double)N[i];
-            }
-            if (fin) {
-                denoised[4 * i + 0] = o[0];
-                denoised[4 * i + 1] = o[1];
-                denoised[4 * i + 2] = o[2];
-                denoised[4 * i + 3] = 1.0;
-            }
-        }
-    }
-}
-
-typedef void (*c_rows_fn)(const c_job*);
-typedef struct c_task {
-    c_rows_fn fn;
-    c_job job;
-} c_task;
-
-static void* c_task_entry(void* arg)
-{
-    c_task* t = (c_task*)arg;
-    t->fn(&t->job);
-    return NULL;
-}
-
-/* Runs fn over all rows, split statically over `threads` pthreads (rows are independent
- * within a stage). */
-static void c_run_rows(c_rows_fn fn, const c_job* base, int threads)
-{
-    if (threads <= 1) {
-        c_job j = *base;
-        j.y0 = 0;
-        j.y1 = base->h;
-        fn(&j);
-        return;
-    }
-    c_task tasks[256];
-    pthread_t tid[256];
-    for (int t = 0; t < threads; ++t) {
-        tasks[t].fn = fn;
-        tasks[t].job = *base;
-        tasks[t].job.y0 = (int)((int64_t)base->h * t / threads);
-        tasks[t].job.y1 = (int)((int64_t)base->h * (t + 1) / threads);
-        pthread_create(&tid[t], NULL, c_task_entry, &tasks[t]);
-    }
-    for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
-}
-
-int oracle_c_frame_f64_mt(int w, int h, const oracle_params* p, int history_valid,
-                          const float* radiance, const float* gbuffer, const float* gbuffer_prev,
-                          const float* motion, const double* history_prev, const double* moments_prev,
-                          double* history_curr, double* moments_curr, double* denoised,
-                          double* dbg_temporal, double* dbg_variance, int threads)
-{
-    if (w <= 0 || h <= 0 || !p || p->iterations < 1 || p->iterations > 16) return 1;
-    if (!radiance || !history_curr || !moments_curr || !denoised) return 1;
-    const int as_ref = (p->flags & ORACLE_FLAG_AS_REFERENCE) != 0;
-    const int comp = (p->flags & ORACLE_FLAG_COMPENSATED) != 0 && !as_ref;
-    if (!as_ref && (!gbuffer || !motion)) return 1;
-    if (history_valid && (!moments_prev || (!as_ref && (!history_prev || !gbuffer_prev)))) return 1;
-    if (threads < 1) threads = 1;
-    if (threads > h) threads = h;
-    if (threads > 256) threads = 256;
-    const size_t np = (size_t)w * h;
-    double* C = (double*)malloc(np * 4 * sizeof(double));
-    int* N = (int*)malloc(np * sizeof(int));
-    double* WA = (double*)malloc(np * 4 * sizeof(double));
-    double* WB = (double*)malloc(np * 4 * sizeof(double));
-    if (!C || !N || !WA || !WB) {
-        free(C);
-        free(N);
-        free(WA);
-        free(WB);
-        return 2;
-    }
-    c_job J;
-    memset(&J, 0, sizeof J);
-    J.w = w;
-    J.h = h;
-    J.history_valid = history_valid;
-    J.as_ref = as_ref;
-    J.comp = comp;
-    J.p = p;
-    J.radiance = radiance;
-    J.gbuffer = gbuffer;
-    J.gbuffer_prev = gbuffer_prev;
-    J.motion = motion;
-    J.history_prev = history_prev;
-    J.moments_prev = moments_prev;
-    J.history_curr = history_curr;
-    J.moments_curr = moments_curr;
-    J.denoised = denoised;
-    J.C = C;
-    J.WA = WA;
-    J.WB = WB;
-    J.N = N;
-
-    /* ---- temporal ---- */
-    c_run_rows(c_temporal_rows, &J, threads);
-    if (dbg_temporal) {
-        for (size_t i = 0; i < np; ++i) {
-            dbg_temporal[4 * i + 0] = C[4 * i + 0];
-            dbg_temporal[4 * i + 1] = C[4 * i + 1];
-            dbg_temporal[4 * i + 2] = C[4 * i + 2];
-            dbg_temporal[4 * i + 3] = as_ref ? C[4 * i + 3] : (double)N[i];
-        }
-    }
-
-    /* ---- variance ---- */
-    c_run_rows(c_variance_rows, &J, threads);
-    if (dbg_variance) memcpy(dbg_variance, WA, np * 4 * sizeof(double));
-
-    /* ---- a-trous ---- */
-    J.src = WA;
-    J.dst = WB;
-    for (int k = 0; k < (int)p->iterations; ++k) {
-        J.k = k;
-        c_run_rows(c_atrous_rows, &J, threads);
-        const double* t = J.src;
-        J.src = J.dst;
-        J.dst = (double*)t;
-    }
-    free(C);
-    free(N);
-    free(WA);
-    free(WB);
-    return 0;
-}
-
-int oracle_c_frame_f64(int w, int h, const oracle_params* p, int history_valid,
-                       const float* radiance, const float* gbuffer, const float* gbuffer_prev,
-                       const float* motion, const double* history_prev, const double* moments_prev,
-                       double* history_curr, double* moments_curr, double* denoised,
-                       double* dbg_temporal, double* dbg_variance)
-{
-    return oracle_c_frame_f64_mt(w, h, p, history_valid, radiance, gbuffer, gbuffer_prev, motion, history_prev,
-                                 moments_prev, history_curr, moments_curr, denoised, dbg_temporal, dbg_variance, 1);
-}
+#define CREAL float
+#define CSUF(name) name##_f32
+#define C_FRAME oracle_c_frame_f32
+#define C_FRAME_MT oracle_c_frame_f32_mt
+#define C_LIT(x) ((float)(x))
+#define C_EXP expf
+#define C_POW powf
+#define C_SQRT sqrtf
+#define C_FABS fabsf
+#define C_FLOOR floorf
+#define C_FMAX fmaxf
+#include "svgf_oracle_canon.inc"

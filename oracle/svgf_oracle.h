@@ -86,6 +86,18 @@ int oracle_c_frame_f64_mt(int w, int h, const oracle_params* p, int history_vali
                           const float* motion, const double* history_prev, const double* moments_prev,
                           double* history_curr, double* moments_curr, double* denoised,
                           double* dbg_temporal, double* dbg_variance, int threads);
+/* The same CANONICAL code in binary32 (float state planes, float arithmetic): the CPU baseline
+ * port of the FP32 canonical form. Not an oracle; pinned within 2e-4 of the binary64 one. */
+int oracle_c_frame_f32(int w, int h, const oracle_params* p, int history_valid,
+                       const float* radiance, const float* gbuffer, const float* gbuffer_prev,
+                       const float* motion, const float* history_prev, const float* moments_prev,
+                       float* history_curr, float* moments_curr, float* denoised,
+                       float* dbg_temporal, float* dbg_variance);
+int oracle_c_frame_f32_mt(int w, int h, const oracle_params* p, int history_valid,
+                          const float* radiance, const float* gbuffer, const float* gbuffer_prev,
+                          const float* motion, const float* history_prev, const float* moments_prev,
+                          float* history_curr, float* moments_curr, float* denoised,
+                          float* dbg_temporal, float* dbg_variance, int threads);
 
 /* Agent-frame wire encoder (wire_oracle.c): docs/AI_INTERFACE.md:346-383 layout, colour through
  * postprocess.slang's fitted ACES + sRGB, quantised by decision thresholds. */

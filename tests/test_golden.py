@@ -114,3 +114,21 @@ PLANTED = {
 def test_fixtures_see_a_changed_rule(can, oracle_lib, name):
     worst, lengths = _canonical_errors(can, "comp", _planted(PLANTED[name]))
     assert worst > 1e-6 or not lengths, (name, worst)
+
+
+@pytest.mark.parametrize("form", ["raw", "comp"])
+def test_float_port_close_to_fixtures(can, oracle_lib, form):
+    """The binary32 build of the CANONICAL code (the bench's CPU baseline port, float state
+    carried from frame to frame) within 5e-5 of the FP64 fixtures, history lengths exact."""
+    worst = 0.0
+    for i, (w, h) in enumerate(can["shapes"]):
+        hist = mom = gp = None
+        for f in range(can[f"s{i}_radiance"].shape[0]):
+            o = O.c_frame_f32(can[f"s{i}_radiance"][f], can[f"s{i}_gbuffer"][f], gp, can[f"s{i}_motion"][f], hist, mom,
+                              O.canonical_params(5, compensated=form == "comp"), history_valid=f > 0)
+            want_h = can[f"s{i}_{form}_history"][f]
+            assert np.array_equal(o["history"][..., 3], want_h[..., 3]), (i, f)
+            worst = max(worst, _rel(o["history"][..., :3], want_h[..., :3]), _rel(o["denoised"], can[f"s{i}_{form}_denoised"][f]),
+                        _rel(o["moments"], can[f"s{i}_{form}_moments"][f]))
+            hist, mom, gp = o["history"], o["moments"], can[f"s{i}_gbuffer"][f]
+    assert worst <= 5e-5, worst
