@@ -128,6 +128,23 @@ def test_create_rejects_bad_descriptors():
     assert L.svgf_destroy(None) == 1
 
 
+@pytest.mark.parametrize("field,value", [
+    ("iterations", 0), ("iterations", 11), ("phi_luma", 0.0), ("phi_luma", float("nan")), ("phi_luma", -1.0),
+    ("phi_normal", -1.0), ("phi_normal", float("nan")), ("phi_normal", float("inf")), ("phi_depth", -0.5),
+    ("alpha_color", -0.1), ("alpha_color", 1.5), ("alpha_moments", 1.01), ("alpha_moments", float("nan")),
+    ("variance_eps", 0.0), ("variance_eps", -1e-4), ("depth_eps", 0.0), ("depth_reject", float("nan")),
+    ("normal_reject", 1.5), ("normal_reject", float("nan")), ("history_cap", 0), ("history_cap", 256),
+    ("flags", 0x4)])
+def test_create_rejects_each_bad_canonical_parameter(field, value):
+    """Every CANONICAL parameter outside its domain is SVGF_ERR_INVALID_ARGUMENT before any
+    device is touched (FP32 storage, so the history cap may go up to 255)."""
+    p = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP32)
+    setattr(p, field, value)
+    d = S.Desc(S.MODE_CANONICAL, S.STORAGE_FP32, p, 64, 64, 0, 0, 0, 0, 0, 0)
+    h = ctypes.c_void_p()
+    assert S.lib().svgf_create(ctypes.byref(d), ctypes.byref(h)) == 1 and not h.value
+
+
 def test_null_context_calls_fail_cleanly():
     L = S.lib()
     sid = ctypes.c_uint32()
