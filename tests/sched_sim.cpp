@@ -143,7 +143,8 @@ int main(int argc, char** argv)
             n, [&](uint32_t i, uint32_t& ww, uint32_t& hh) { ww = w[i]; hh = h[i]; }, canonical, stages, B.hdr,
             B.starts.data());
         B.total = (uint32_t)total;
-        B.hdr[HDR_CHUNK] = 1u + rng() % 5u;
+        // chunks of 1..5 tickets mostly, and up to 64 now and then
+        B.hdr[HDR_CHUNK] = b % 3 == 2 ? 1u + rng() % 64u : 1u + rng() % 5u;
         // decode_ticket against a direct enumeration
         {
             uint32_t t = 0;
