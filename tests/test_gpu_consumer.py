@@ -81,3 +81,47 @@ def test_compiled_host_vs_oracle(S, oracle_lib, tmp_path, exe, mode, storage):
         assert rel_err(mom[..., 0], ref[-1]["moments"][..., 0])[0] <= bound
     assert rel_err(h3, ref[-1]["history"][..., :3])[0] <= bound
     assert np.array_equal(n, ref[-1]["history"][..., 3])
+
+
+@pytest.mark.parametrize("storage", [0, 1], ids=["fp32", "fp16_wide"])
+def test_compiled_batch_host_vs_oracle(S, oracle_lib, tmp_path, storage):
+    """svgf_batch_consumer (C++20): three resident CANONICAL streams of uneven sizes, one batch per
+    frame with every request keeping its G-buffer (two planes per stream, alternating), each
+    batch's outcome read with svgf_batch_status, the final state read back through
+    svgf_stream_state_get. Every stream against the FP64 oracle: FP32 within 2e-4, the FP16
+    precision form within 1e-3; history lengths exact."""
+    O = oracle_lib
+    path = BUILD / "svgf_batch_consumer"
+    assert path.exists(), f"{path} is not built (make -C tests/consumer, run by __graft_entry__.build())"
+    fp16 = storage == 1
+    dt = np.float16 if fp16 else np.float32
+    shapes, frames = [(96, 64), (130, 70), (200, 120)], 4
+    inputs = [seq_inputs(w, h, frames, seed0=40 + 10 * i, stream=i, noise=1 if fp16 else 0, pan_px=1.1,
+                         disocclusion=0.1, half=fp16) for i, (w, h) in enumerate(shapes)]
+    for i, seq in enumerate(inputs):
+        for f, d in enumerate(seq):
+            d["radiance"].astype(dt).tofile(tmp_path / f"s{i}_radiance_{f}.bin")
+            d["gbuffer"].tofile(tmp_path / f"s{i}_gbuffer_{f}.bin")
+            d["motion"].tofile(tmp_path / f"s{i}_motion_{f}.bin")
+    args = [str(path), str(storage), str(frames), str(tmp_path)] + [str(v) for wh in shapes for v in wh]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    bound = 1e-3 if fp16 else 2e-4
+    for i, (w, h) in enumerate(shapes):
+        ref = oracle_canonical_sequence(inputs[i], O.canonical_params(5, compensated=fp16), threads=8)
+        for f in range(frames):
+            den = np.fromfile(tmp_path / f"s{i}_denoised_{f}.bin", dt).reshape(h, w, 4).astype(np.float64)
+            assert rel_err(den[..., :3], ref[f]["denoised"][..., :3])[0] <= bound, (i, f)
+        hist = np.fromfile(tmp_path / f"s{i}_history.bin", dt).reshape(h, w, 4)
+        if fp16:
+            h3, n = S.decode_history(hist)
+            mom = S.decode_moments(np.fromfile(tmp_path / f"s{i}_moments.bin", dt).reshape(h, w, 4))
+            me = moment_errors(mom, ref[-1]["moments"])
+            assert me["mean"][0] <= bound and me["second"][0] <= bound, (i, me)
+        else:
+            h3, n = hist[..., :3], hist[..., 3]
+            mom = np.fromfile(tmp_path / f"s{i}_moments.bin", dt).reshape(h, w, 2)
+            assert rel_err(mom, ref[-1]["moments"])[0] <= bound, i
+        assert rel_err(h3, ref[-1]["history"][..., :3])[0] <= bound, i
+        assert np.array_equal(n, ref[-1]["history"][..., 3]), i
