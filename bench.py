@@ -56,12 +56,13 @@ def stage_bytes(storage_bytes4, storage_bytes2, iterations, g_copy):
     return [temporal] + atrous
 
 
-# Stage-structured compulsory bytes per pixel of the survey's canonical FP16 layout (R, history,
-# moments, wavelets binary16; G and motion FP32; no gradient plane): temporal 72, variance 36,
-# 5 a-trous iterations of 32 (wavelet 8 + G 16 read, wavelet 8 written), iteration 0's history
-# write 8 = 276 per frame. The roofline reports its fraction with these bytes beside this layout's.
-SURVEY_FP16_ATROUS_BYTES_PX = 32
-SURVEY_FP16_FRAME_BYTES_PX = 276
+# Stage-structured compulsory bytes per pixel of the canonical pass with every colour, state and
+# wavelet plane in binary16 (R, history, moments and wavelets binary16; G and motion FP32; no
+# gradient plane): temporal 72, variance 36, 5 a-trous iterations of 32 (wavelet 8 + G 16 read,
+# wavelet 8 written), iteration 0's history write 8 = 276 per frame. The roofline reports its
+# fraction with these bytes beside this layout's.
+FP16_ATROUS_BYTES_PX = 32
+FP16_FRAME_BYTES_PX = 276
 
 
 def _rel(a, b, floor=1e-3):
@@ -761,13 +762,13 @@ def main():
     # Dominant kernel (largest share of the batch's GPU time): the residue-pair a-trous kernel of
     # the iterations before the last (strides 2..2^(N-2): one code object, N-2 launches per
     # batch; the last iteration is another kernel, which skips the filtered variance). `achieved`
-    # uses the survey's per-pixel bytes of an FP16 a-trous iteration (32 B/px) times the pixels
+    # uses the all-binary16 per-pixel bytes of an a-trous iteration (32 B/px) times the pixels
     # of one launch, over the kernel's mean launch duration from the event nodes of the timed
     # batches' own graphs; this layout's bytes (FP32 wavelets, gradient plane) are given beside.
     at = slice(2, p16.iterations)
     dom_ms = float(np.mean(stage_ms[at])) if stage_ms is not None else float("nan")
     layout_bytes_px = float(np.mean(sb16[at]))
-    achieved = SURVEY_FP16_ATROUS_BYTES_PX * batch_px / (dom_ms * 1e-3) / 1e9
+    achieved = FP16_ATROUS_BYTES_PX * batch_px / (dom_ms * 1e-3) / 1e9
     layout_achieved = layout_bytes_px * batch_px / (dom_ms * 1e-3) / 1e9
     bytes_px_16 = sum(sb16)
     bytes_px_32 = sum(stage_bytes(16, 8, p32.iterations, g_copy=False))
@@ -989,7 +990,7 @@ def main():
                         "reads that G-buffer in place as its previous one (the single-frame form likewise reads "
                         "the caller's previous G-buffer)" if args.gbuffer == "kept" else
                         "copied: the temporal stage saves a copy of G in the stream's state for the next frame"),
-            "survey_bytes_per_px": SURVEY_FP16_FRAME_BYTES_PX,
+            "fp16_planes_bytes_per_px": FP16_FRAME_BYTES_PX,
         },
         "accuracy": accuracy,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
@@ -1017,9 +1018,9 @@ def main():
             "ratio_over_plain_fp32": {"target": 3.0, "measured": round(batched_mpx / plain_mpx, 3),
                                       "met": batched_mpx / plain_mpx >= 3.0},
             "hbm_peak_fraction": {"target": 0.6,
-                                  "measured": round(batched_mpx * 1e6 * SURVEY_FP16_FRAME_BYTES_PX / 1e9 / HBM_PEAK_GBPS, 4),
+                                  "measured": round(batched_mpx * 1e6 * FP16_FRAME_BYTES_PX / 1e9 / HBM_PEAK_GBPS, 4),
                                   "measured_layout_bytes": round(batched_mpx * 1e6 * bytes_px_16 / 1e9 / HBM_PEAK_GBPS, 4),
-                                  "met": batched_mpx * 1e6 * SURVEY_FP16_FRAME_BYTES_PX / 1e9 / HBM_PEAK_GBPS >= 0.6},
+                                  "met": batched_mpx * 1e6 * FP16_FRAME_BYTES_PX / 1e9 / HBM_PEAK_GBPS >= 0.6},
             "max_rel_err_vs_fp64": {"target": 1e-3,
                                     "measured": None if accuracy is None else max(accuracy["max_rel_err"].values()),
                                     "met": None if accuracy is None else accuracy["within_bound"]},
@@ -1029,7 +1030,7 @@ def main():
                    "and faster kernels speed the plain FP32 form as much as the batched one",
         },
         "effective_GBps": round(batched_mpx * 1e6 * bytes_px_16 / 1e9, 1),
-        "effective_GBps_survey_bytes": round(batched_mpx * 1e6 * SURVEY_FP16_FRAME_BYTES_PX / 1e9, 1),
+        "effective_GBps_fp16_planes_bytes": round(batched_mpx * 1e6 * FP16_FRAME_BYTES_PX / 1e9, 1),
         "stage_ms": None if stage_ms is None else {n: round(float(t), 5) for n, t in zip(names, stage_ms)},
         "roofline": {"bound": "hbm",
                      "kernel": f"k_batch<F16S, ST_CAN_ATROUS_PAIR, SF_WIDE>: a-trous iterations 1..{p16.iterations - 2} "
@@ -1041,8 +1042,8 @@ def main():
                                         "profiles/traffic.txt: rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per "
                                         "launch of this kernel on this workload (committed figure, not measured "
                                         "in this run)" if args.traffic is None else "--traffic argument"),
-                     "algorithmic_bytes_per_px": SURVEY_FP16_ATROUS_BYTES_PX,
-                     "algorithmic_bytes_per_launch": SURVEY_FP16_ATROUS_BYTES_PX * batch_px,
+                     "algorithmic_bytes_per_px": FP16_ATROUS_BYTES_PX,
+                     "algorithmic_bytes_per_launch": FP16_ATROUS_BYTES_PX * batch_px,
                      "pixels_per_launch": batch_px,
                      "layout_bytes_per_px": layout_bytes_px,
                      "layout_achieved": round(layout_achieved, 1),

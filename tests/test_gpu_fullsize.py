@@ -13,7 +13,7 @@ from svgf_testutil import (MARGIN_16, crop_interior, crop_windows, decode_state,
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
-THREADS = min(16, os.cpu_count() or 1)  # oracle threads (the GPU box gives one GPU 16 cores)
+THREADS = min(16, os.cpu_count() or 1)  # oracle threads: at most 16, one GPU's share of the host
 
 
 @pytest.fixture(scope="module")
