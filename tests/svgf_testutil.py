@@ -109,7 +109,7 @@ def gpu_canonical_sequence(ctx, inputs, dtype, crops=None):
 # history (reprojection 2 + variance 1 + first pass 2) 5 more: under 80 in all. On the
 # 640x520 oracle sequence of the locality test the farthest pixel a crop edge changes is 68.
 MARGIN = 128
-# The same after 16 frames at config 2's pan (3.74 px/frame at 3840x2160), lognormal noise and
+# The same after 16 frames at the 4K precision sequence's pan (3.74 px/frame at 3840x2160), lognormal noise and
 # 10% disocclusion: the farthest pixel a crop edge changes is 127 (denoised), 78 (history),
 # 59 (moments) on a 1100x700 oracle sequence; 192 leaves a third more.
 MARGIN_16 = 192

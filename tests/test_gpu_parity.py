@@ -39,7 +39,7 @@ def _ref_gpu(S, ctx, lighting, moments_prev, dtype, history_valid):
 
 
 @pytest.mark.parametrize("iterations", [3, 5])
-def test_reference_fp32_config0(S, oracle_lib, iterations):
+def test_reference_fp32_640x360_warm_frame(S, oracle_lib, iterations):
     """640x360: history = a frame from seed 1, current frame from seed 0 (alpha 0.2)."""
     O = oracle_lib
     w, h = 640, 360

@@ -1,12 +1,12 @@
 """GPU tests at the sizes of the batched and serving workloads.
 
-Config 3: the 64-frame mix (resolutions drawn uniformly with seed 0 from {96x96, 128x128,
+Batched: the 64-frame mix (resolutions drawn uniformly with seed 0 from {96x96, 128x128,
 256x256, 320x180, 640x480, 1280x720}, each stream with its own seeded generator and resident
 history), 3 batches, in the per-stage graph form and the persistent form: every stream's
 output is bit-identical to the single-frame path, and two streams of every class match the
 FP64 oracle within 1e-3 max relative error.
 
-Config 4: the serving load, 256 concurrent streams, Poisson arrivals at 4000 requests/s,
+Serving: 256 concurrent streams, Poisson arrivals at 4000 requests/s,
 5 %/s stream churn (cold history), a 64 GB HBM budget, the same resolution mix: every request
 completes, and the requests of sampled streams give the same bits as their stream denoised
 alone through the single-frame path.
@@ -41,7 +41,7 @@ def _bits(t):
     return t.view(torch.int16 if t.dtype == torch.float16 else torch.int32).cpu()
 
 
-def test_config3_mix64(S, oracle_lib):
+def test_batched_mix64(S, oracle_lib):
     O = oracle_lib
     dev = torch.device("cuda")
     rng = np.random.default_rng(0)
@@ -56,7 +56,7 @@ def test_config3_mix64(S, oracle_lib):
     dev_in = [[{"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
                 "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
                 "motion": torch.from_numpy(d["motion"]).to(dev)} for d in fr] for fr in host]
-    print(f"config 3: {len(shapes)} frames, {npx} px per batch")
+    print(f"batched mix: {len(shapes)} frames, {npx} px per batch")
 
     # single-frame path, stream by stream
     alone = []
@@ -121,12 +121,12 @@ def test_config3_mix64(S, oracle_lib):
             me = moment_errors(m2, o["moments"])
             worst["moments_mean"] = max(worst.get("moments_mean", 0.0), me["mean"][0])
             worst["moments_second"] = max(worst.get("moments_second", 0.0), me["second"][0])
-    print("config 3 vs FP64 oracle, max rel err:", {k: f"{v:.3e}" for k, v in worst.items()})
+    print("batched mix vs FP64 oracle, max rel err:", {k: f"{v:.3e}" for k, v in worst.items()})
     for k, v in worst.items():
         assert v <= 1e-3, (k, v)
 
 
-def test_config4_serving_load(S):
+def test_serving_256_streams(S):
     dev = torch.device("cuda")
     rate, duration, streams, churn = 4000.0, 1.5, 256, 0.05
     rng = np.random.default_rng(4)
@@ -196,7 +196,7 @@ def test_config4_serving_load(S):
         completed = len(statuses)
     used = ctx.budget_used()
     ctx.close()
-    print(f"config 4: {submitted} requests in {duration} s ({submitted / duration:.0f}/s), {reopened} streams "
+    print(f"serving: {submitted} requests in {duration} s ({submitted / duration:.0f}/s), {reopened} streams "
           f"reopened, {used / 2**30:.2f} GiB of stream state")
     assert submitted > 0.8 * rate * duration
     assert completed == submitted and all(s == 0 for s in statuses)
