@@ -32,13 +32,16 @@ def S():
 SHAPES = [(96, 96), (128, 128), (33, 17), (256, 64), (70, 300), (1, 1)]
 
 
-@pytest.mark.parametrize("storage", ["fp16", "fp32", "fp16_persistent"])
+@pytest.mark.parametrize("storage", ["fp16", "fp32", "fp16_persistent", "fp16_kept_gbuffer"])
 def test_server_results_equal_alone(S, storage):
     """fp16_persistent: each batch is one persistent launch, and with two lanes two such
-    launches share the device."""
+    launches share the device. fp16_kept_gbuffer: every request keeps its G-buffer
+    (SVGF_REQUEST_KEEP_GBUFFER; the inputs outlive the run), so each stream's next frame, in
+    whichever lane and batch it lands, reads it in place."""
     st = S.STORAGE_FP32 if storage == "fp32" else S.STORAGE_FP16
     dt = torch.float32 if storage == "fp32" else torch.float16
     flags = S.BATCH_PERSISTENT if storage.endswith("persistent") else 0
+    req_flags = S.REQUEST_KEEP_GBUFFER if storage.endswith("kept_gbuffer") else 0
     dev = torch.device("cuda")
     frames = 4
     inputs = [seq_inputs(w, h, frames, seed0=100 + 7 * i, stream=i, pan_px=0.8, disocclusion=0.1)
@@ -63,7 +66,7 @@ def test_server_results_equal_alone(S, storage):
                 items[(i, f)] = {"stream_id": sids[i], "radiance": torch.from_numpy(d["radiance"]).to(dev).to(dt),
                                  "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
                                  "motion": torch.from_numpy(d["motion"]).to(dev),
-                                 "denoised": torch.empty((h, w, 4), dtype=dt, device=dev)}
+                                 "denoised": torch.empty((h, w, 4), dtype=dt, device=dev), "flags": req_flags}
                 outs[(i, f)] = items[(i, f)]["denoised"]
             torch.cuda.synchronize()
             # random interleaving that keeps each stream's frame order; bursts of several
