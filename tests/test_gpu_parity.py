@@ -123,16 +123,32 @@ def test_reference_fp16_moment_overflow_and_nan_taps(S, oracle_lib):
             assert mx <= 2.0 ** -10 + 1e-6, f
 
 
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
 @pytest.mark.parametrize("shape", [(1, 1), (2, 9), (9, 2), (65, 5), (67, 131)])
-def test_reference_edge_sizes(S, oracle_lib, shape):
+def test_reference_edge_sizes(S, oracle_lib, shape, storage):
+    """Frames smaller than the taps' reach (every tap clamped), a cold and a warm frame: FP32
+    within 1e-5 of the float restatement; FP16 storage within one half ulp of the half-store
+    restatement, moments bit for bit."""
     O = oracle_lib
     w, h = shape
-    cur = S.synth_frame(w, h, seed=3, frame=0)["radiance"]
-    with S.Context(S.MODE_REFERENCE, S.STORAGE_FP32, max_width=w, max_height=h) as ctx:
-        d, m = _ref_gpu(S, ctx, cur, None, torch.float32, False)
-    om, od, _, _ = O.r_frame_f32(cur, None, O.reference_params(3), history_valid=False)
-    assert rel_err(d, od)[0] <= 1e-5
-    assert rel_err(m, om)[0] <= 1e-5
+    fp16 = storage == "fp16"
+    st, dt = (S.STORAGE_FP16, torch.float16) if fp16 else (S.STORAGE_FP32, torch.float32)
+    seq = [S.synth_frame(w, h, seed=3 + f, frame=f)["radiance"] for f in range(2)]
+    if fp16:
+        seq = [O.round_f16(L) for L in seq]
+    om = None
+    with S.Context(S.MODE_REFERENCE, st, max_width=w, max_height=h) as ctx:
+        m = None
+        for f, L in enumerate(seq):
+            d, m = _ref_gpu(S, ctx, L, m, dt, f > 0)
+            om, od, _, _ = O.r_frame_f32(L, om, O.reference_params(3), store=O.STORE_F16 if fp16 else O.STORE_F32,
+                                         history_valid=f > 0)
+            if fp16:
+                assert rel_err(d, od)[0] <= 2.0 ** -10 + 1e-6, f
+                assert np.array_equal(m, om), f
+            else:
+                assert rel_err(d, od)[0] <= 1e-5, f
+                assert rel_err(m, om)[0] <= 1e-5, f
 
 
 def test_canonical_fp32_sequence(S, oracle_lib):
