@@ -38,7 +38,7 @@ def _ref_gpu(S, ctx, lighting, moments_prev, dtype, history_valid):
     return den.float().cpu().numpy(), mom.float().cpu().numpy()
 
 
-@pytest.mark.parametrize("iterations", [3, 5])
+@pytest.mark.parametrize("iterations", [1, 3, 5, 10])
 def test_reference_fp32_640x360_warm_frame(S, oracle_lib, iterations):
     """640x360: history = a frame from seed 1, current frame from seed 0 (alpha 0.2)."""
     O = oracle_lib
@@ -248,6 +248,46 @@ def test_canonical_edge_stop_settings(S, oracle_lib, phis):
         assert dm <= 2e-4 and dmean <= 1e-6, f
 
 
+@pytest.mark.parametrize("form", ["fp32", "fp16_wide"])
+@pytest.mark.parametrize("iterations,cap", [(1, 32), (2, 1), (4, 63), (7, 8), (10, 32)],
+                         ids=["it1", "it2_cap1", "it4_cap63", "it7_cap8", "it10"])
+def test_canonical_iterations_and_history_caps(S, oracle_lib, form, iterations, cap):
+    """A-trous iteration counts other than 5 change which kernels run: with 1 the first
+    iteration (tall tiles, young-pixel variance) writes the output; with 2 the last iteration is
+    the first residue-pair stage; 7 and 10 reach strides 64 and 512 (lattices of a few points
+    or one, most taps clamped). History caps: 1 (no accumulation: alpha 1), 8, and 63, the
+    largest the wide state encodes. 5 frames of 150x100 vs the FP64 oracle, history lengths
+    exact; FP32 within 2e-4, the FP16 wide form within 1e-3 max relative error."""
+    O = oracle_lib
+    w, h = 150, 100
+    fp16 = form == "fp16_wide"
+    inputs = seq_inputs(w, h, 5, seed0=70, noise=1 if fp16 else 0, pan_px=1.37, disocclusion=0.1, half=fp16)
+    op = O.canonical_params(iterations, compensated=fp16)
+    op.history_cap = cap
+    storage, dt = (S.STORAGE_FP16, torch.float16) if fp16 else (S.STORAGE_FP32, torch.float32)
+    gp = S.default_params(S.MODE_CANONICAL, storage)
+    gp.iterations, gp.history_cap = iterations, cap
+    ref = oracle_canonical_sequence(inputs, op)
+    bound = 1e-3 if fp16 else 2e-4
+    with S.Context(S.MODE_CANONICAL, storage, params=gp, max_width=w, max_height=h) as ctx:
+        assert ctx.wide == fp16
+        got = gpu_canonical_sequence(ctx, inputs, dt)
+    for f, (g, r) in enumerate(zip(got, ref)):
+        dm, dmean = rel_err(g["denoised"][..., :3], r["denoised"][..., :3])
+        hm, _ = rel_err(g["history"][..., :3], r["history"][..., :3])
+        print(f"{form} it={iterations} cap={cap} frame {f}: denoised max {dm:.3e} mean {dmean:.3e}; "
+              f"history max {hm:.3e}")
+        assert np.isfinite(g["denoised"]).all(), f
+        assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), f
+        assert g["history"][..., 3].max() <= cap, f
+        assert dm <= bound and hm <= bound, f
+        if fp16:
+            me = moment_errors(g["moments"], r["moments"])
+            assert me["mean"][0] <= bound and me["second"][0] <= bound, f
+    if cap > 5:
+        assert got[-1]["history"][..., 3].max() == 5  # every frame valid: lengths reach 5
+
+
 @pytest.mark.parametrize("wide", [True, False], ids=["wide_state", "plain_fp16_state"])
 def test_canonical_fp16_compensated_sequence(S, oracle_lib, wide):
     """Precision form: FP16 storage, compensated moments, lognormal noise, 16 frames, vs the
@@ -321,8 +361,24 @@ def test_batch_matches_single(S, mode, launch):
     """Each stream's result is bit-identical alone (single-frame form) and in a batch, for the
     per-stage launches and the persistent launch, replayed from a graph or launched directly.
     5 batches on 3 internal slots: the persistent form reuses a slot's counters."""
+    _batch_vs_single(S, mode, launch)
+
+
+@pytest.mark.parametrize("mode", ["reference", "canonical"])
+@pytest.mark.parametrize("launch", ["graph", "persistent_graph"])
+@pytest.mark.parametrize("iterations", [1, 10])
+def test_batch_matches_single_iteration_limits(S, mode, launch, iterations):
+    """The same at the smallest and largest a-trous iteration counts: 2 stages (the first
+    a-trous iteration writes the output) and 11 (strides up to 512, wider than every frame)."""
+    _batch_vs_single(S, mode, launch, iterations)
+
+
+def _batch_vs_single(S, mode, launch, iterations=None):
     m = S.MODE_REFERENCE if mode == "reference" else S.MODE_CANONICAL
     st = S.STORAGE_FP16
+    params = S.default_params(m, st)
+    if iterations is not None:
+        params.iterations = iterations
     dt = torch.float16
     dev = torch.device("cuda")
     shapes = _ragged_shapes()
@@ -334,7 +390,7 @@ def test_batch_matches_single(S, mode, launch):
     npx = sum(w * h for w, h in shapes)
     flags = (S.BATCH_PERSISTENT if launch.startswith("persistent") else 0) | \
         (S.BATCH_NO_GRAPH if launch.endswith("direct") else 0)
-    with S.Context(m, st, max_width=maxw, max_height=maxh, max_streams=8, max_batch_frames=8,
+    with S.Context(m, st, params=params, max_width=maxw, max_height=maxh, max_streams=8, max_batch_frames=8,
                    max_batch_pixels=npx, batch_flags=flags) as bctx:
         sids = [bctx.stream_open(w, h) for w, h in shapes]
         batched = [[None] * frames for _ in shapes]
@@ -354,7 +410,7 @@ def test_batch_matches_single(S, mode, launch):
             for i, it in enumerate(items):
                 batched[i][f] = it["denoised"]
         torch.cuda.synchronize()
-    with S.Context(m, st, max_width=maxw, max_height=maxh) as sctx:
+    with S.Context(m, st, params=params, max_width=maxw, max_height=maxh) as sctx:
         for i, (w, h) in enumerate(shapes):
             if m == S.MODE_CANONICAL:
                 alone = gpu_canonical_sequence(sctx, inputs[i], dt)
@@ -611,3 +667,36 @@ def test_batch_outcomes_outlive_their_slots(S):
         torch.cuda.synchronize()
         assert ctx.batch_status(first, wait=True) == 1  # evicted: invalid argument
         assert ctx.batch_status(ctx.batch_last(), wait=True) == 0
+
+
+def test_batch_stage_tile_limit(S):
+    """Without the persistent form each stage is one launch of one block per tile, and a launch
+    holds at most 2^32 - 1 work-items (16777215 blocks of 256 threads). At 10 iterations a
+    CANONICAL frame has 131072 tiles at stride 512 whatever its size: a batch of 128 frames is
+    refused with SVGF_ERR_CAPACITY before anything is enqueued (no batch number, every stream
+    still at frame 0), one of 127 runs, and each stream's output equals the single-frame
+    form's bit for bit."""
+    dev = torch.device("cuda")
+    p = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
+    p.iterations = 10
+    n, w, h = 128, 2, 1
+    frames = [S.synth_frame(w, h, seed=300 + i, frame=0) for i in range(n)]
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p, max_width=w, max_height=h, max_streams=n,
+                   max_batch_frames=n, max_batch_pixels=n * w * h) as ctx:
+        sids = [ctx.stream_open(w, h) for _ in range(n)]
+        items = [{"stream_id": sid, "radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                  "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev), "motion": torch.from_numpy(d["motion"]).to(dev),
+                  "denoised": torch.empty((h, w, 4), dtype=torch.float16, device=dev)} for sid, d in zip(sids, frames)]
+        with pytest.raises(S.SvgfError) as e:
+            ctx.denoise_batch(items)
+        assert e.value.status == 6  # SVGF_ERR_CAPACITY
+        assert ctx.batch_last() == 0
+        assert all(ctx.stream_state(sid).frames == 0 for sid in sids)
+        ctx.denoise_batch(items[:n - 1])
+        assert ctx.batch_status(ctx.batch_last(), wait=True) == 0
+        assert all(ctx.stream_state(sid).frames == (1 if i < n - 1 else 0) for i, sid in enumerate(sids))
+        torch.cuda.synchronize()
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p, max_width=w, max_height=h) as sctx:
+        for i in (0, 1, 63, n - 2):
+            alone = gpu_canonical_sequence(sctx, [frames[i]], torch.float16)
+            assert np.array_equal(alone[0]["denoised"], items[i]["denoised"].float().cpu().numpy()), i

@@ -990,6 +990,19 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
         arena_need += frame_scratch_bytes(c->w4,(uint64_t)s->w * s->h);
     }
     if (total_px > c->desc.max_batch_pixels || arena_need > c->arena_bytes) return SVGF_ERR_CAPACITY;
+    if (!persistent(c)) {
+        // one block of BLOCK threads per tile of a stage, and a launch holds at most 2^32 - 1
+        // work-items; the tiles of a stride-s stage grow with s^2 (a frame has at least s^2 / 2
+        // tiles at s = 512), so many frames at 9 or 10 iterations can exceed that
+        for (int st = 0; st < stage_count(c); ++st) {
+            uint64_t t = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                const StreamSlot& s = c->streams[req[i].stream_id];
+                t += stage_tiles(c->mode == SVGF_MODE_CANONICAL, st, s.w, s.h);
+            }
+            if (t > MAX_LAUNCH_BLOCKS) return SVGF_ERR_CAPACITY;
+        }
+    }
 
     DeviceGuard guard(c->desc.device);
     if (persistent(c)) {

@@ -26,6 +26,8 @@ constexpr int LT = 16;
 constexpr int LHALO = 2;
 constexpr int LREG = LT + 2 * LHALO;  // 20
 constexpr int BLOCK = LT * LT;        // 256 threads
+// Blocks of one launch: its work-items (blocks x BLOCK) must fit in 32 bits.
+constexpr uint32_t MAX_LAUNCH_BLOCKS = 0xFFFFFFFFu / BLOCK;
 
 SVGF_HD inline uint32_t tiles_at(uint32_t w, uint32_t h, uint32_t log2s)
 {
