@@ -227,6 +227,27 @@ def test_canonical_incoherent_motion(S, oracle_lib, form):
         assert dm <= (1e-3 if fp16 else 2e-4), f
 
 
+@pytest.mark.parametrize("form", ["fp32", "fp16_wide"])
+def test_canonical_zero_motion(S, oracle_lib, form):
+    """A motion plane of zeros is same-pixel history (the reference's temporal rule): 4 frames of
+    a static camera with 10 % forced disocclusion vs the FP64 oracle, history lengths exact, and
+    every pixel that was never disoccluded has length = frame + 1."""
+    O = oracle_lib
+    w, h = 130, 70
+    fp16 = form != "fp32"
+    inputs = seq_inputs(w, h, 4, seed0=90, noise=1 if fp16 else 0, pan_px=0.0, disocclusion=0.1, half=fp16)
+    for d in inputs:
+        d["motion"] = np.zeros_like(d["motion"])
+    ref = oracle_canonical_sequence(inputs, O.canonical_params(5, compensated=fp16))
+    storage, dt = (S.STORAGE_FP16, torch.float16) if fp16 else (S.STORAGE_FP32, torch.float32)
+    with S.Context(S.MODE_CANONICAL, storage, max_width=w, max_height=h) as ctx:
+        got = gpu_canonical_sequence(ctx, inputs, dt)
+    for f, (g, r) in enumerate(zip(got, ref)):
+        assert np.array_equal(g["history"][..., 3], r["history"][..., 3]), f
+        assert g["history"][..., 3].max() == f + 1, f
+        assert rel_err(g["denoised"][..., :3], r["denoised"][..., :3])[0] <= (1e-3 if fp16 else 2e-4), f
+
+
 @pytest.mark.parametrize("phis", [(4.0, 0.0, 0.0), (4.0, 8.0, 0.5), (2.0, 128.0, 0.0), (10.0, 0.0, 3.0)],
                          ids=["luma_only", "weak_normal", "no_depth", "no_normal"])
 def test_canonical_edge_stop_settings(S, oracle_lib, phis):

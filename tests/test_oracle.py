@@ -388,15 +388,17 @@ def test_canonical_cold_equals_full_disocclusion():
     assert np.all(cold["temporal"][..., 3] == 1)
 
 
-def test_canonical_integer_pan_reprojects_exactly():
-    """Motion (+1, 0) with a consistent G-buffer: the temporal colour is
-    H[y, x+1] + a (R - H[y, x+1]) with a = max(0.2, 1/n), n = n_hist + 1."""
+@pytest.mark.parametrize("shift", [0, 1])
+def test_canonical_integer_pan_reprojects_exactly(shift):
+    """Motion (+shift, 0) with a consistent G-buffer: the temporal colour is
+    H[y, x+shift] + a (R - H[y, x+shift]) with a = max(0.2, 1/n), n = n_hist + 1. Zero motion is
+    same-pixel history (the reference's own temporal rule, which has no reprojection)."""
     rng = np.random.default_rng(14)
     h, w = 6, 10
     L = rand_img(rng, h, w)
     g = _flat_g(h, w)
     mv = np.zeros((h, w, 2), np.float32)
-    mv[..., 0] = 1.0
+    mv[..., 0] = shift
     hist = rng.random((h, w, 4)) * 3
     hist[..., 3] = np.array([1, 2, 3, 9, 40, 1, 2, 3, 4, 5])[None, :]
     mom = rng.random((h, w, 2)) * 3
@@ -404,13 +406,14 @@ def test_canonical_integer_pan_reprojects_exactly():
     out = O.c_frame_f64(L, g, g, mv, hist, mom, p, history_valid=True, debug=True)
     t = out["temporal"]
     for y in range(h):
-        for x in range(w - 1):
-            n = min(int(hist[y, x + 1, 3]) + 1, 32)
+        for x in range(w - shift):
+            n = min(int(hist[y, x + shift, 3]) + 1, 32)
             a = max(float(np.float32(0.2)), 1.0 / n)
-            H = hist[y, x + 1, :3]
+            H = hist[y, x + shift, :3]
             np.testing.assert_allclose(t[y, x, :3], H + a * (L[y, x, :3] - H), rtol=1e-14)
             assert t[y, x, 3] == n
-        assert t[y, w - 1, 3] == 1  # reprojects outside the image: disoccluded
+        if shift:
+            assert t[y, w - 1, 3] == 1  # reprojects outside the image: disoccluded
 
 
 def test_canonical_raw_and_compensated_moments_agree():
