@@ -20,6 +20,7 @@ import ctypes
 import json
 import os
 import pathlib
+import statistics
 import sys
 import time
 
@@ -865,31 +866,45 @@ def main():
         threads = max(1, min(16, os.cpu_count() or 1))
         op = O.canonical_params(p16.iterations, compensated=True)
 
-        def cpu_rate(frame_fn, seconds):
-            # warm-history frames of the 64-frame mix, in stream order, until `seconds` of work
-            done_px, t_cpu, frames_done = 0, 0.0, 0
-            for i, (w, h) in enumerate(shapes):
-                d0, d1 = host[i][0], host[i][1]
-                a = frame_fn(d0["radiance"], d0["gbuffer"], None, d0["motion"], None, None, op, history_valid=False,
-                             threads=threads)
-                t0 = time.perf_counter()
-                frame_fn(d1["radiance"], d1["gbuffer"], d0["gbuffer"], d1["motion"], a["history"], a["moments"],
-                         op, history_valid=True, threads=threads)
-                t_cpu += time.perf_counter() - t0
-                done_px += w * h
-                frames_done += 1
-                if t_cpu >= seconds:
-                    break
-            return done_px / t_cpu / 1e6, frames_done, done_px
+        def cpu_rate(frame_fn, seconds, passes=5):
+            # warm-history frames of the 64-frame mix, in stream order: the first pass takes frames
+            # until seconds / passes of work, the others time the same frames; the median pass
+            colds, n_frames, rates = {}, None, []
+            for _ in range(passes):
+                done_px, t_cpu, frames_done = 0, 0.0, 0
+                for i, (w, h) in enumerate(shapes):
+                    if n_frames is not None and i == n_frames:
+                        break
+                    d0, d1 = host[i][0], host[i][1]
+                    if i not in colds:
+                        colds[i] = frame_fn(d0["radiance"], d0["gbuffer"], None, d0["motion"], None, None, op,
+                                            history_valid=False, threads=threads)
+                    a = colds[i]
+                    t0 = time.perf_counter()
+                    frame_fn(d1["radiance"], d1["gbuffer"], d0["gbuffer"], d1["motion"], a["history"], a["moments"],
+                             op, history_valid=True, threads=threads)
+                    t_cpu += time.perf_counter() - t0
+                    done_px += w * h
+                    frames_done += 1
+                    if n_frames is None and t_cpu >= seconds / passes:
+                        break
+                n_frames = frames_done
+                rates.append(done_px / t_cpu / 1e6)
+            return statistics.median(rates), n_frames, done_px, rates
 
-        rate, frames_done, done_px = cpu_rate(O.c_frame_f32, args.cpu_seconds)
-        rate64, frames64, px64 = cpu_rate(O.c_frame_f64, args.cpu_seconds / 2)
+        rate, frames_done, done_px, rates = cpu_rate(O.c_frame_f32, args.cpu_seconds)
+        rate64, frames64, px64, _ = cpu_rate(O.c_frame_f64, args.cpu_seconds / 2, passes=3)
+        cpu_model = next((ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")),
+                         "unknown")
         cpu = {"value": round(rate, 4), "unit": "Mpx/s", "cores": threads, "kind": "port",
                "sample": f"{frames_done} warm-history frames of the 64-frame mix ({done_px} px), CANONICAL in "
                          f"binary32 (the oracle's C code built for float, compensated moments; rows split over "
-                         f"{threads} threads)",
+                         f"{threads} threads); median of {len(rates)} passes over the same frames",
+               "passes_mpx_per_s": [round(r, 4) for r in rates],
+               "host": {"cpu_model": cpu_model, "logical_cpus": os.cpu_count(),
+                        "usable_cpus": len(os.sched_getaffinity(0))},
                "oracle_fp64": {"mpx_per_s": round(rate64, 4), "frames": frames64, "pixels": px64,
-                               "what": "the same frames through the binary64 oracle"}}
+                               "what": "the same frames through the binary64 oracle (median of 3 passes)"}}
         # the reference's own semantics on its CPU path: REFERENCE-mode float restatement, 1080p
         L1080 = S.synth_frame(1920, 1080, seed=0, frame=1, pan_px=0.5)["radiance"]
         M1080 = O.r_frame_f32(L1080, None, O.reference_params(3), history_valid=False, threads=threads)[0]
