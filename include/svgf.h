@@ -254,7 +254,9 @@ SVGF_API svgf_status svgf_budget_used(svgf_context* ctx, uint64_t* bytes);
  * observed (by svgf_batch_status, or by a later svgf_denoise_batch / svgf_stream_close call)
  * its streams restart with cold history, so no later frame reprojects the wrong state. In the
  * persistent form a call whose requests name a stream of a batch still in flight first waits
- * for that batch on the host. SVGF_ERR_CAPACITY: more requests or pixels than svgf_desc allows,
+ * for that batch on the host; in the per-stage form, when that batch was enqueued on another
+ * hip_stream, this call's hip_stream waits for it on the device (hipStreamWaitEvent), so the
+ * frames of a stream run in call order whatever streams the calls use. SVGF_ERR_CAPACITY: more requests or pixels than svgf_desc allows,
  * or (without SVGF_BATCH_PERSISTENT) a stage with more than 16777215 tiles, which only strides
  * of 256 and 512 reach: a CANONICAL frame has at least 32768 tiles at 9 iterations and 131072
  * at 10 (REFERENCE: twice as many), so keep such batches under 512 and 128 frames (256 and 64). */

@@ -630,6 +630,48 @@ def test_single_frame_calls_on_two_streams(S, mode):
                     assert torch.equal(a[k].view(torch.int16), b[k].view(torch.int16)), k
 
 
+@pytest.mark.parametrize("launch", ["graph", "direct"])
+def test_batches_of_one_stream_on_alternating_hip_streams(S, launch):
+    """Consecutive batches naming the same agent streams, enqueued back to back on two HIP
+    streams in turn without a host synchronisation: each batch reads the state the previous one
+    writes, so the library orders them on the device. Every output equals the same sequence on
+    one HIP stream, bit for bit."""
+    dev = torch.device("cuda")
+    dt = torch.float16
+    shapes = [(1280, 720), (640, 480), (96, 96)]
+    frames = 6
+    inputs = [seq_inputs(w, h, frames, seed0=40 + 10 * i, stream=i, pan_px=0.6, disocclusion=0.05, half=True)
+              for i, (w, h) in enumerate(shapes)]
+    npx = sum(w * h for w, h in shapes)
+    flags = S.BATCH_NO_GRAPH if launch == "direct" else 0
+
+    def run(streams):
+        with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=1280, max_height=720, max_streams=4,
+                       max_batch_frames=4, max_batch_pixels=npx, batch_flags=flags) as ctx:
+            sids = [ctx.stream_open(w, h) for w, h in shapes]
+            items_all = []
+            for f in range(frames):
+                items = []
+                for i, (w, h) in enumerate(shapes):
+                    d = inputs[i][f]
+                    items.append({"stream_id": sids[i], "radiance": torch.from_numpy(d["radiance"]).to(dev).to(dt),
+                                  "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                                  "motion": torch.from_numpy(d["motion"]).to(dev),
+                                  "denoised": torch.empty((h, w, 4), dtype=dt, device=dev)})
+                items_all.append(items)
+            torch.cuda.synchronize()
+            for f in range(frames):
+                ctx.denoise_batch(items_all[f], stream=streams[f % len(streams)])
+            torch.cuda.synchronize()
+            return [[it["denoised"].view(torch.int16).cpu() for it in items] for items in items_all]
+
+    one = run([torch.cuda.Stream()])
+    two = run([torch.cuda.Stream(), torch.cuda.Stream()])
+    for f in range(frames):
+        for i in range(len(shapes)):
+            assert torch.equal(one[f][i], two[f][i]), (f, shapes[i])
+
+
 def test_set_params_keeps_the_state_form_of_open_streams(S):
     """The moment / history form (compensated, wide) cannot change while resident streams hold
     state in the old one; other parameters can. With no stream open the form changes, and the

@@ -71,6 +71,7 @@ struct BatchSlot {
     hipEvent_t sev[MAX_STAGES + 1]{};  // SVGF_BATCH_TIMED_STAGES: events around the batch's stage launches
     int sev_n = 0;                      // stages those events bracket in the slot's last batch
     uint64_t seq = 0;        // id of the batch the slot last ran (0: none)
+    hipStream_t stream = nullptr;  // the hip_stream it was enqueued on
     bool checked = false;    // its outcome has been read (after completion)
     bool failed = false;     // a dependency wait of its persistent launch gave up
 };
@@ -1019,6 +1020,17 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
         }
     }
     if (poll_slots(c) != SVGF_OK) return SVGF_ERR_HIP;
+    hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+    // A batch in flight on another hip_stream that names one of these streams writes the state
+    // this batch reads (and reads the parity this batch writes): this stream waits for it on
+    // the device. Batches on one hip_stream are ordered by the stream itself.
+    for (auto& sl : c->slots) {
+        if (!sl.pending || sl.stream == stream) continue;
+        bool shares = false;
+        for (uint32_t i = 0; i < n && !shares; ++i)
+            shares = std::find(sl.ids.begin(), sl.ids.end(), req[i].stream_id) != sl.ids.end();
+        if (shares && hipStreamWaitEvent(stream, sl.done, 0) != hipSuccess) return SVGF_ERR_HIP;
+    }
     BatchSlot& slot = c->slots[c->next_slot];
     c->next_slot = (c->next_slot + 1) % BATCH_SLOTS;
     if (slot.pending) {
@@ -1066,7 +1078,6 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
     uint32_t tiles[MAX_STAGES];
     for (int st = 0; st < nstages; ++st) tiles[st] = starts[(size_t)st * STARTS_ROW + n];
 
-    hipStream_t stream = static_cast<hipStream_t>(hip_stream);
     hipError_t e = hipSuccess;
     if (c->desc.batch_flags & (SVGF_BATCH_NO_GRAPH | SVGF_PROFILE_STAGES)) {
         e = enqueue_batch(c, slot, tiles, stream);
@@ -1086,6 +1097,7 @@ svgf_status svgf_denoise_batch(svgf_context* c, const svgf_request* req, uint32_
     if (e != hipSuccess) return SVGF_ERR_HIP;
     if (hipEventRecord(slot.done, stream) != hipSuccess) return SVGF_ERR_HIP;
     slot.pending = true;
+    slot.stream = stream;
     slot.seq = ++c->last_seq;
     slot.checked = false;
     slot.failed = false;
