@@ -66,6 +66,53 @@ FP16_ATROUS_BYTES_PX = 32
 FP16_FRAME_BYTES_PX = 276
 
 
+CALIB_OPS = ["fma", "exp2", "log2", "rcp", "sqrt", "exp2_fma_1_1", "exp2_fma_1_7"]
+
+
+def valu_rates(ops=CALIB_OPS):
+    """Sustained wave64 VALU issue rates of cuda:0 per instruction mix (tron-grid_amd/lib/
+    libsvgf_calib.so: 16 independent chains per thread, 8 waves per SIMD on every CU), or None
+    when the calibration library is absent."""
+    import ctypes
+
+    path = ROOT / "tron-grid_amd" / "lib" / "libsvgf_calib.so"
+    if not path.exists():
+        return None
+    L = ctypes.CDLL(str(path))
+    L.svgf_calib_valu_op.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for name in ops:
+        r = ctypes.c_double()
+        if L.svgf_calib_valu_op(0, CALIB_OPS.index(name), ctypes.byref(r)) != 0:
+            return None
+        out[name] = r.value
+    return out
+
+
+def valu_calibration():
+    """The sustained issue rate a VALU-bound kernel reaches on this device (FMA chains, and one
+    v_exp_f32 in eight instructions as the a-trous taps mix them), for the roofline line."""
+    r = valu_rates(["fma", "exp2_fma_1_7"])
+    if r is None:
+        return None
+    nominal = 256 * 4 * 2.4e9 / 4
+    return {"fma_wave_instr_per_s": round(r["fma"], -6), "exp_1_in_8_wave_instr_per_s": round(r["exp2_fma_1_7"], -6),
+            "fma_cycles_per_instr_at_2p4GHz": round(4 * nominal / r["fma"], 3),
+            "what": "best of 5 launches of 16 independent v_fma_f32 chains per thread, 8 waves per SIMD on "
+                    "every CU (and the same with one chain in eight on v_exp_f32): the issue rate pure FMA "
+                    "work sustains on this device under load (the clock drops below 2.4 GHz)"}
+
+
+def valu_report() -> dict:
+    """Issue cost of each instruction kind relative to v_fma_f32 on this device."""
+    r = valu_rates()
+    if r is None:
+        return {"error": "tron-grid_amd/lib/libsvgf_calib.so is not built"}
+    nominal = 256 * 4 * 2.4e9 / 4
+    return {k: {"wave_instr_per_s": round(v, -6), "cycles_per_instr_at_2p4GHz": round(4 * nominal / v, 3),
+                "time_relative_to_fma": round(r["fma"] / v, 3)} for k, v in r.items()}
+
+
 def _rel(a, b, floor=1e-3):
     import numpy as np
 
@@ -514,10 +561,11 @@ def main():
                     help="HBM bytes per a-trous launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE (reported as is)")
     ap.add_argument("--no-ref720", dest="ref720", action="store_false",
                     help="skip the REFERENCE-mode 1280x720 timing (the reference's published case)")
-    ap.add_argument("--report", choices=["precision", "copy", "composite", "pack"], default=None,
+    ap.add_argument("--report", choices=["precision", "copy", "composite", "pack", "valu"], default=None,
                     help="precision: error of the GPU forms vs the FP64 oracle on 16-frame 1080p / 4K sequences; "
                          "copy: one 1 GiB device copy (counter calibration); composite / pack: the consumer composite / "
-                         "the producer pack only")
+                         "the producer pack only; valu: sustained issue rates of FMA, transcendental and mixed "
+                         "instruction chains on this device")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched form (for profiler runs)")
     ap.add_argument("--launch", choices=["persistent", "per_stage"], default="per_stage",
@@ -528,6 +576,9 @@ def main():
     ap.add_argument("--report-threads", type=int, default=16)
     args = ap.parse_args()
 
+    if args.report == "valu":
+        print(json.dumps({"valu_rates": valu_report()}))
+        return
     if args.report == "pack":
         print(json.dumps({"producer_pack": pack_report(args.composite_reps, not args.no_cpu_baseline)}))
         return
@@ -565,6 +616,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.Stream(device=dev)
+    calib = valu_calibration()
 
     rng = np.random.default_rng(0)
     shapes = [RESOLUTIONS[i] for i in rng.integers(0, len(RESOLUTIONS), args.frames)]
@@ -761,12 +813,13 @@ def main():
     names = ["temporal"] + [f"atrous{k}" for k in range(p16.iterations)]
     sb16 = stage_bytes(8, 8 if p16.flags & S.FLAG_WIDE_STATE else 4, p16.iterations, g_copy=args.gbuffer == "copied")
     # Dominant kernel (largest share of the batch's GPU time): the residue-pair a-trous kernel of
-    # the iterations before the last (strides 2..2^(N-2): one code object, N-2 launches per
-    # batch; the last iteration is another kernel, which skips the filtered variance). `achieved`
-    # uses the all-binary16 per-pixel bytes of an a-trous iteration (32 B/px) times the pixels
-    # of one launch, over the kernel's mean launch duration from the event nodes of the timed
-    # batches' own graphs; this layout's bytes (FP32 wavelets, gradient plane) are given beside.
-    at = slice(2, p16.iterations)
+    # strides 2 and 4 (iterations 1 and 2: one code object, two launches per batch; stride 8 and
+    # the last iteration are other kernels). `achieved` uses the all-binary16 per-pixel bytes of
+    # an a-trous iteration (32 B/px) times the pixels of one launch, over the kernel's mean
+    # launch duration from the event nodes of the timed batches' own graphs; this layout's
+    # bytes (FP32 wavelets, gradient plane) are given beside.
+    n_dom = max(0, min(2, p16.iterations - 2))
+    at = slice(2, 2 + n_dom)
     dom_ms = float(np.mean(stage_ms[at])) if stage_ms is not None else float("nan")
     layout_bytes_px = float(np.mean(sb16[at]))
     achieved = FP16_ATROUS_BYTES_PX * batch_px / (dom_ms * 1e-3) / 1e9
@@ -924,12 +977,13 @@ def main():
         # HBM bytes and VALU instructions per a-trous launch, measured with rocprofv3 (FETCH_SIZE /
         # WRITE_SIZE, SQ_INSTS_VALU) on this workload and recorded beside the profile summaries
         for ln in tfile.read_text().splitlines():
-            if ln.startswith("atrous_pair_wide_batched_fp16_mix64_bytes_per_launch:") and traffic is None:
+            if ln.startswith("atrous_pair_s2s4_wide_batched_fp16_mix64_bytes_per_launch:") and traffic is None:
                 traffic = float(ln.split(":", 1)[1])
-            if ln.startswith("atrous_pair_wide_batched_fp16_mix64_valu_insts_per_launch:"):
+            if ln.startswith("atrous_pair_s2s4_wide_batched_fp16_mix64_valu_insts_per_launch:"):
                 valu_insts = float(ln.split(":", 1)[1])
     # VALU issue rate of the chip: one wave64 instruction per SIMD every 4 cycles (16 lanes per
-    # cycle), 4 SIMDs per CU, 256 CUs, at the 2.4 GHz peak engine clock
+    # cycle), 4 SIMDs per CU, 256 CUs, at the 2.4 GHz peak engine clock; beside it the sustained
+    # rate measured on this device in this run (calibration kernels of FMA chains)
     valu_issue_per_s = 256 * 4 * 2.4e9 / 4
 
     progress("accuracy of the timed batched output")
@@ -1040,17 +1094,18 @@ def main():
                                     "measured": None if accuracy is None else max(accuracy["max_rel_err"].values()),
                                     "met": None if accuracy is None else accuracy["within_bound"]},
             "p99_reported": True,
-            "why": "the a-trous iterations issue ~16 VALU instructions per tap (2 transcendental) for 25 taps a "
-                   "pixel, at 75-83 % VALU issue: arithmetic, not HBM, bounds every stage but the temporal one, "
-                   "and faster kernels speed the plain FP32 form as much as the batched one",
+            "why": "the a-trous iterations issue 16 VALU instructions per tap (2 transcendental) for 25 taps "
+                   "a pixel; the dominant kernel issues VALU instructions as fast as pure FMA chains do on this "
+                   "device (roofline.valu), so arithmetic, not HBM, bounds every stage but the temporal one, and "
+                   "faster kernels speed the plain FP32 form as much as the batched one",
         },
         "effective_GBps": round(batched_mpx * 1e6 * bytes_px_16 / 1e9, 1),
         "effective_GBps_fp16_planes_bytes": round(batched_mpx * 1e6 * FP16_FRAME_BYTES_PX / 1e9, 1),
         "stage_ms": None if stage_ms is None else {n: round(float(t), 5) for n, t in zip(names, stage_ms)},
         "roofline": {"bound": "hbm",
-                     "kernel": f"k_batch<F16S, ST_CAN_ATROUS_PAIR, SF_WIDE>: a-trous iterations 1..{p16.iterations - 2} "
-                               f"(strides {', '.join(str(1 << k) for k in range(1, p16.iterations - 1))}; "
-                               f"{p16.iterations - 2} launches per batch)",
+                     "kernel": f"k_batch<F16S, ST_CAN_ATROUS_PAIR_E, SF_WIDE>: the residue-pair a-trous "
+                               f"iterations at strides {', '.join(str(1 << k) for k in range(1, 1 + n_dom))} "
+                               f"({n_dom} launches per batch)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "traffic_source": (None if traffic is None else
@@ -1067,13 +1122,18 @@ def main():
                      "launch_timing": "hipEvents recorded by event nodes around each stage launch inside the "
                                       "graphs of the timed batches (SVGF_BATCH_TIMED_STAGES), on the launch "
                                       "stream; mean over the kernel's launches in the timed region",
-                     "note": "arithmetic-bound: ~16 VALU instructions (2 transcendental) per tap, 25 taps per "
-                             "pixel, with the VALU and LDS the busiest units; HBM fraction is not the binding limit",
+                     "note": "VALU-issue-bound: 16 VALU instructions (2 transcendental, whose issue cost the "
+                             "FMAs around them hide) per tap, 25 taps per pixel; the HBM fraction is not the "
+                             "binding limit",
                      "valu": None if valu_insts is None or not np.isfinite(dom_ms) else {
                          "instructions_per_launch": int(valu_insts),
+                         "achieved_wave_instr_per_s": round(valu_insts / (dom_ms * 1e-3), -6),
                          "issue_frac": round(valu_insts / (dom_ms * 1e-3) / valu_issue_per_s, 4),
                          "peak": "one wave64 VALU instruction per SIMD per 4 cycles, 1024 SIMDs, 2.4 GHz "
                                  "(a lower clock under load raises the fraction)",
+                         "sustained": calib,
+                         "issue_frac_sustained": None if calib is None else round(
+                             valu_insts / (dom_ms * 1e-3) / calib["fma_wave_instr_per_s"], 4),
                          "source": "profiles/traffic.txt: rocprofv3 SQ_INSTS_VALU per launch of this kernel on "
                                    "this workload (committed figure) over this run's mean launch time"}},
         "cpu_baseline": cpu,

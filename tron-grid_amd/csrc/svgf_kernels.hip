@@ -358,19 +358,29 @@ __device__ __forceinline__ void ref_atrous_tile(const FrameDesc& f, const KParam
     const Buf src = bind(source_is_a ? f.wa : f.wb, f.h);
     const int max_x = f.w - 1, max_y = f.h - 1;
     const int tid = lane_tid();
-    for (int e = tid; e < LREG * LREG; e += BLOCK) {
-        int li = e % LREG, lj = e / LREG;
-        int qx = clampi(T.px(T.i0 - LHALO + li), 0, max_x), qy = clampi(T.py(T.j0 - LHALO + lj), 0, max_y);
-        float4 v = S::ld4(src, qx, qy);
-        sA[e] = make_float4(v.x, v.y, v.z, luma_ref(v.x, v.y, v.z));
-    }
-    __syncthreads();
     const int ti = tid & (LT - 1), tj = tid >> 4;
     const int x = T.px(T.i0 + ti), y = T.py(T.j0 + tj);
+    // The centre's sigma (alpha) and every entry of the region are loaded before the first LDS
+    // write: one memory round trip. The last pass's surplus lanes load the final entry again.
+    const float cw = S::lda(src, min(x, max_x), min(y, max_y));
+    constexpr int NE = LREG * LREG, NIT = (NE + BLOCK - 1) / BLOCK;
+    float4 v[NIT];
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+        const int e = min(tid + j * BLOCK, NE - 1);
+        const int li = e % LREG, lj = e / LREG;
+        const int qx = clampi(T.px(T.i0 - LHALO + li), 0, max_x), qy = clampi(T.py(T.j0 - LHALO + lj), 0, max_y);
+        v[j] = S::ld4(src, qx, qy);
+    }
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) asm volatile("" ::"v"(v[j].x), "v"(v[j].y), "v"(v[j].z));
+#pragma unroll
+    for (int j = 0; j < NIT; ++j)
+        if (tid + j * BLOCK < NE) sA[tid + j * BLOCK] = make_float4(v[j].x, v[j].y, v[j].z, luma_ref(v[j].x, v[j].y, v[j].z));
+    __syncthreads();
     if (x <= max_x && y <= max_y) {
         const int c0 = (tj + LHALO) * LREG + ti + LHALO;
         float4 c = sA[c0];
-        float cw = S::lda(src, x, y);
         float lc = c.w;
         float sigma = fmaxf(p.phi_luma * cw, p.variance_eps);
         float nrs = -1.0f / sigma;
@@ -422,14 +432,14 @@ __device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(
 // variance)) into wavelet_a, the history length (uint8) and the moments; the batched form
 // also keeps G for the next frame. Standard deviations rather than variances are stored so
 // that heavy-tailed radiance stays inside the FP16 range.
-// Pixel (x, y) of the frame, with its G and the depths of its four clamped neighbours.
+// Pixel (x, y) of the frame, with its G, radiance R, motion m (unused without history) and the
+// depths of its four clamped neighbours.
 template <class S, int SF>
 __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KParams& p, int x, int y, float4 G,
-                                                   float zl, float zr, float zu, float zd)
+                                                   float4 R, float2 m, float zl, float zr, float zu, float zd)
 {
     constexpr bool COMP = SF != SF_RAW, WIDE = SF == SF_WIDE;
     static_assert(!WIDE || S::B4 == 8, "the wide state is a binary16 form");
-    float4 R = S::ld4(bind(f.rad, f.h), x, y);
     float l = luma_fma(R.x, R.y, R.z);
     if (f.gsave.ptr) stg4(bind(f.gsave, f.h), x, y, G);
     {
@@ -445,8 +455,7 @@ __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KPa
     float tb[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tmu[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tm2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     int nmax = 0;
     if (f.history_valid) {
-        const Buf mvb = bind(f.mv, f.h), gp = bind(f.gprev, f.h), hp = bind(f.hist_prev, f.h), mp = bind(f.mom_prev, f.h);
-        float2 m = ldg2(mvb, x, y);
+        const Buf gp = bind(f.gprev, f.h), hp = bind(f.hist_prev, f.h), mp = bind(f.mom_prev, f.h);
         float flx = floorf(m.x), fly = floorf(m.y);
         float fx = m.x - flx, fy = m.y - fly;
         int x0 = x + (int)flx, y0 = y + (int)fly;
@@ -577,10 +586,13 @@ __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KPar
     // lookups read.
     constexpr int ZW = LT + 2;
     float* const sZ = reinterpret_cast<float*>(L::template get<TEMPORAL_LDS>());
-    const float4 G = ldg4(gb, min(x, max_x), min(y, max_y));
-    sZ[(tj + 1) * ZW + ti + 1] = G.w;
-    if (tid < 4 * LT + 4) {
-        // border: the rows above and below (ZW entries each), then the columns left and right
+    const int cx = min(x, max_x), cy = min(y, max_y);
+    const float4 G = ldg4(gb, cx, cy);
+    // border: the rows above and below (ZW entries each), then the columns left and right
+    const bool border = tid < 4 * LT + 4;
+    int bslot = 0;
+    float zb = 0.0f;
+    if (border) {
         int bx, by;
         if (tid < 2 * ZW) {
             bx = tid % ZW - 1;
@@ -590,13 +602,21 @@ __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KPar
             bx = (c & 1) ? LT : -1;
             by = c >> 1;
         }
-        sZ[(by + 1) * ZW + bx + 1] = ldgz(gb, clampi(T.i0 + bx, 0, max_x), clampi(T.j0 + by, 0, max_y));
+        bslot = (by + 1) * ZW + bx + 1;
+        zb = ldgz(gb, clampi(T.i0 + bx, 0, max_x), clampi(T.j0 + by, 0, max_y));
     }
+    // The pixel's radiance and motion are loaded with G, before the barrier (whose fence keeps
+    // later loads behind it): the border depths, G, R and the motion then take one memory round
+    // trip together, and the reprojection gathers a second.
+    const float4 R = S::ld4(bind(f.rad, f.h), cx, cy);
+    const float2 m = f.history_valid ? ldg2(bind(f.mv, f.h), cx, cy) : make_float2(0.0f, 0.0f);
+    sZ[(tj + 1) * ZW + ti + 1] = G.w;
+    if (border) sZ[bslot] = zb;
     lds_writes_done();
     __syncthreads();
     if (x <= max_x && y <= max_y) {
         const int c = (tj + 1) * ZW + ti + 1;
-        can_temporal_pixel<S, SF>(f, p, x, y, G, sZ[c - 1], sZ[c + 1], sZ[c - ZW], sZ[c + ZW]);
+        can_temporal_pixel<S, SF>(f, p, x, y, G, R, m, sZ[c - 1], sZ[c + 1], sZ[c - ZW], sZ[c + ZW]);
     }
     __syncthreads();  // the next tile of a grid-stride loop reuses sZ
 }
@@ -756,10 +776,11 @@ struct Sums {
 // LAST: the caller's k is iterations - 1 >= 1; the output carries no variance, so the taps
 // skip the squared-weight sum and its LDS reads. SF: the moment / history form (the wide form
 // encodes the colour history written by FIRST).
-template <class S, int SF, bool FIRST, bool LAST, int NR, bool TALL, class L>
+template <class S, int SF, bool FIRST, bool LAST, int NR, bool TALL, bool EARLY, class L>
 __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParams& p, const Tile& T, int k)
 {
     static_assert(!FIRST || NR == 1, "the first iteration runs at stride 1");
+    static_assert(!FIRST || EARLY, "the first iteration loads its history lengths with the staged region");
     static_assert(!(FIRST && LAST), "the first iteration keeps its runtime last-iteration test");
     static_assert(!TALL || FIRST, "tall tiles are the first iteration's");
     constexpr bool BLK = TALL || NR == 2;
@@ -784,9 +805,35 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     // staged region (no extra round trip)
     constexpr int NQ = FIRST ? (QW * QH + BLOCK - 1) / BLOCK : 0;
     uint32_t nq[NQ > 0 ? NQ : 1];
+    // The thread's lattice points: (x, y0) and, in blocked tiles, (x, y1) with y1 = y0 + s.
+    // Their depth gradients (strided iterations: the temporal stage's plane; iteration 0 takes
+    // them from the staged depths) or history lengths (iteration 0) are loaded at coordinates
+    // clamped into the frame (the values of in-frame points). EARLY: together with the staged
+    // region, whose loads all complete before its first LDS write, one memory round trip for
+    // the whole tile. Otherwise after the barrier: at strides of 8 and more every lane's
+    // gradient is a cache line of its own, and loading those lines beside the staging loads
+    // (which there touch a line per entry) delays the staging more than the round trip costs
+    // (measured; at strides 1 to 4 the early form is faster).
+    const Buf gr = bind(f.grad, f.h), nl = bind(f.nlen, f.h);
+    // residue res (NR = 2: tid / 128) and lattice point (ti, tj) of the tile
+    const int res = NR == 2 ? tid >> 7 : 0, lt = NR == 2 ? tid & 127 : tid, ti = lt & (LT - 1);
+    const int tj = BLK ? 2 * (lt >> 4) : lt >> 4;
+    const int x = T.px(T.i0 + ti) + res, y0 = T.py(T.j0 + tj), y1 = y0 + s;
+    const int cx = min(x, max_x), cy0 = min(y0, max_y), cy1 = min(y1, max_y);
+    float2 g0 = make_float2(0.0f, 0.0f), g1 = make_float2(0.0f, 0.0f);
+    uint32_t n0 = 0, n1 = 0;
+    auto load_point_terms = [&] {
+        if constexpr (FIRST) {
+            n0 = ld_u8(nl, cy0 * nl.pitch + cx);
+            if constexpr (BLK) n1 = ld_u8(nl, cy1 * nl.pitch + cx);
+        } else {
+            g0 = ld_f2(gr, cy0 * gr.pitch + cx * 8);
+            if constexpr (BLK) g1 = ld_f2(gr, cy1 * gr.pitch + cx * 8);  // unused past the frame
+        }
+    };
+    if constexpr (EARLY) load_point_terms();
     if constexpr (FIRST) {
         if (tid == 0) sCount = 0;
-        const Buf nl = bind(f.nlen, f.h);
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
             const int e = tid + j * BLOCK;
@@ -812,6 +859,14 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             const int qy = clampi(T.ry + ((by + (int)lj) << T.log2s), 0, max_y);
             va[j] = WavS::ld4(src, qx, qy);
             vb[j] = ldg4(gb, qx, qy);
+        }
+        // EARLY: the loaded values are needed here; otherwise the optimiser sinks the last
+        // pass's loads into its conditional store, a second memory round trip
+        if constexpr (EARLY) {
+#pragma unroll
+            for (uint32_t j = 0; j < NIT; ++j)
+                asm volatile("" ::"v"(va[j].x), "v"(va[j].y), "v"(va[j].z), "v"(va[j].w), "v"(vb[j].x),
+                             "v"(vb[j].y), "v"(vb[j].z), "v"(vb[j].w));
         }
 #pragma unroll
         for (uint32_t j = 0; j < NIT; ++j) {
@@ -870,12 +925,8 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
                 gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sV[c + dy * RW + dx], gv);
         return gv;
     };
-    // the per-pixel depth gradient the temporal stage stored
-    const Buf gr = bind(f.grad, f.h);
-    // FIRST also writes the colour history: the output with the pixel's history length, loaded
-    // before the taps so that its latency hides behind them
-    const Buf nl = bind(f.nlen, f.h);
-    auto length = [&](int x, int y) { return FIRST ? ld_u8(nl, y * nl.pitch + x) : 0u; };
+    if constexpr (!EARLY) load_point_terms();
+    // FIRST also writes the colour history: the output with the pixel's history length
     auto store = [&](int x, int y, float4 o, uint32_t n) {
         if constexpr (FIRST) {
             if constexpr (SF == SF_WIDE) st_wide_history(bind(f.hist_curr, f.h), x, y, o, n);
@@ -885,14 +936,13 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
         else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
     };
     if constexpr (!BLK) {
-        const int ti = tid & (LT - 1), tj = tid >> 4;
-        const int x = T.px(T.i0 + ti), y = T.py(T.j0 + tj);
+        const int y = y0;
         if (x <= max_x && y <= max_y) {
             const int c0 = (tj + H) * RW + ti + H;
             const float4 C = sA[c0];
             const float2 g = FIRST ? make_float2(0.5f * (sB[c0 + 1].w - sB[c0 - 1].w), 0.5f * (sB[c0 + RW].w - sB[c0 - RW].w))
-                                   : ld_f2(gr, y * gr.pitch + x * 8);
-            const uint32_t n = length(x, y);
+                                   : g0;
+            const uint32_t n = n0;
             const Centre c = centre_of(p, C, sB[c0], prefilter(c0), g.x, g.y, s);
             Sums<!LAST> a;
 #pragma unroll
@@ -908,23 +958,16 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             store(x, y, a.result(C, sV[c0]), n);
         }
     } else {
-        // residue r (NR = 2: tid / 128); lattice points (ti, 2u) and (ti, 2u + 1) of the tile
-        const int r = NR == 2 ? tid >> 7 : 0, l = NR == 2 ? tid & 127 : tid, ti = l & (LT - 1), u = l >> 4;
-        const int x = T.px(T.i0 + ti) + r, y0 = T.py(T.j0 + 2 * u), y1 = y0 + s;
+        // lattice points (ti, tj) and (ti, tj + 1) of residue res
         if (x <= max_x && y0 <= max_y) {
-            const int c0 = r * RR + (2 * u + H) * RW + ti + H, c1 = c0 + RW;
+            const int c0 = res * RR + (tj + H) * RW + ti + H, c1 = c0 + RW;
             const float4 C0 = sA[c0], C1 = sA[c1];
-            float2 g0, g1;
             if constexpr (FIRST) {
                 // stride 1: the clamped neighbours are staged entries (the temporal stage's
                 // central differences, the same floats)
                 g0 = make_float2(0.5f * (sB[c0 + 1].w - sB[c0 - 1].w), 0.5f * (sB[c1].w - sB[c0 - RW].w));
                 g1 = make_float2(0.5f * (sB[c1 + 1].w - sB[c1 - 1].w), 0.5f * (sB[c1 + RW].w - sB[c0].w));
-            } else {
-                g0 = ld_f2(gr, y0 * gr.pitch + x * 8);
-                g1 = ld_f2(gr, min(y1, max_y) * gr.pitch + x * 8);  // unused past the frame
             }
-            const uint32_t n0 = length(x, y0), n1 = length(x, min(y1, max_y));
             const Centre e0 = centre_of(p, C0, sB[c0], prefilter(c0), g0.x, g0.y, s);
             const Centre e1 = centre_of(p, C1, sB[c1], prefilter(c1), g1.x, g1.y, s);
             Sums<!LAST> a0, a1;
@@ -955,9 +998,15 @@ enum StageId : int {
     ST_REF_INIT = 0, ST_REF_ATROUS = 1, ST_CAN_TEMPORAL = 2, ST_CAN_ATROUS0 = 3, ST_CAN_ATROUS = 4, ST_CAN_ATROUS_PAIR = 5,
     ST_CAN_ATROUS0_SQ = 6,  // the first a-trous iteration on 16x16 tiles (single small frames)
     // the last iteration (k = iterations - 1 >= 1): writes the output, no filtered variance
-    ST_CAN_ATROUS_L = 7, ST_CAN_ATROUS_PAIR_L = 8
+    ST_CAN_ATROUS_L = 7, ST_CAN_ATROUS_PAIR_L = 8,
+    // residue pairs at strides 2 and 4 (not the last iteration): the per-point loads travel
+    // with the staged region
+    ST_CAN_ATROUS_PAIR_E = 9
 };
-constexpr bool is_pair_stage(int st) { return st == ST_CAN_ATROUS_PAIR || st == ST_CAN_ATROUS_PAIR_L; }
+constexpr bool is_pair_stage(int st)
+{
+    return st == ST_CAN_ATROUS_PAIR || st == ST_CAN_ATROUS_PAIR_L || st == ST_CAN_ATROUS_PAIR_E;
+}
 
 template <class S, int STAGE, int SF, class L = OwnLds>
 __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, const Tile& T, int arg)
@@ -965,12 +1014,13 @@ __device__ __forceinline__ void run_stage(const FrameDesc& f, const KParams& p, 
     if constexpr (STAGE == ST_REF_INIT) ref_init_tile<S>(f, p, T);
     else if constexpr (STAGE == ST_REF_ATROUS) ref_atrous_tile<S, L>(f, p, T, arg);
     else if constexpr (STAGE == ST_CAN_TEMPORAL) can_temporal_tile<S, SF, L>(f, p, T);
-    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, SF, true, false, 1, true, L>(f, p, T, 0);
-    else if constexpr (STAGE == ST_CAN_ATROUS0_SQ) can_atrous_tile<S, SF, true, false, 1, false, L>(f, p, T, 0);
-    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, SF, false, false, 1, false, L>(f, p, T, arg);
-    else if constexpr (STAGE == ST_CAN_ATROUS_L) can_atrous_tile<S, SF, false, true, 1, false, L>(f, p, T, arg);
-    else if constexpr (STAGE == ST_CAN_ATROUS_PAIR_L) can_atrous_tile<S, SF, false, true, 2, false, L>(f, p, T, arg);
-    else can_atrous_tile<S, SF, false, false, 2, false, L>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS0) can_atrous_tile<S, SF, true, false, 1, true, true, L>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS0_SQ) can_atrous_tile<S, SF, true, false, 1, false, true, L>(f, p, T, 0);
+    else if constexpr (STAGE == ST_CAN_ATROUS) can_atrous_tile<S, SF, false, false, 1, false, false, L>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS_L) can_atrous_tile<S, SF, false, true, 1, false, false, L>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS_PAIR_L) can_atrous_tile<S, SF, false, true, 2, false, false, L>(f, p, T, arg);
+    else if constexpr (STAGE == ST_CAN_ATROUS_PAIR_E) can_atrous_tile<S, SF, false, false, 2, false, true, L>(f, p, T, arg);
+    else can_atrous_tile<S, SF, false, false, 2, false, false, L>(f, p, T, arg);
 }
 
 // Frame descriptor of a batch entry into scalar registers (the index is uniform across the block).
@@ -1213,6 +1263,7 @@ void canonical_frame(const FrameDesc& f, const KParams& p, hipStream_t s, hipEve
             else launch_frame<S, ST_CAN_ATROUS, SF>(f, p, k, (uint32_t)k, s);
         } else {
             if (last) launch_frame<S, ST_CAN_ATROUS_PAIR_L, SF>(f, p, k, (uint32_t)k, s);
+            else if (k <= 2) launch_frame<S, ST_CAN_ATROUS_PAIR_E, SF>(f, p, k, (uint32_t)k, s);
             else launch_frame<S, ST_CAN_ATROUS_PAIR, SF>(f, p, k, (uint32_t)k, s);
         }
         mark(ev, k + 2, s);
@@ -1246,6 +1297,7 @@ void canonical_batch(const BatchTable& bt, const KParams& p, const uint32_t* g, 
             else launch_batch<S, ST_CAN_ATROUS, SF>(bt, p, k, (uint32_t)k, k + 1, g, s);
         } else {
             if (last) launch_batch<S, ST_CAN_ATROUS_PAIR_L, SF>(bt, p, k, (uint32_t)k, k + 1, g, s);
+            else if (k <= 2) launch_batch<S, ST_CAN_ATROUS_PAIR_E, SF>(bt, p, k, (uint32_t)k, k + 1, g, s);
             else launch_batch<S, ST_CAN_ATROUS_PAIR, SF>(bt, p, k, (uint32_t)k, k + 1, g, s);
         }
         mark(ev, k + 2, s);
