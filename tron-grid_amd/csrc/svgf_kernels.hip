@@ -716,7 +716,9 @@ __device__ __forceinline__ float tap_weight(const KParams& p, const Centre& c, i
     const int o = (dy + 2) * 5 + dx + 2;  // 0..24, centre 12
     float e = log2k(dx + 2, dy + 2);
     if (o != 12) e = fmaf(fabsf(B.w - c.zc), c.nrd[o > 12 ? o - 13 : 11 - o], e);
-    e = fmaf(fabsf(A.w - c.lc), c.nls, e);
+    // the centre's own luminance difference is 0 (or NaN, and then so is every tap's, whose
+    // weights all turn NaN and the result falls back to the centre as it would with it)
+    if (o != 12) e = fmaf(fabsf(A.w - c.lc), c.nls, e);
     // max(0, n.n_q)^phi_n as phi_n log2(d) in the exponent, d clamped to [0, 1] by the dot
     // product's clamp bit: d <= 0 gives log2 = -inf, weight 0 (unit normals exceed 1 only by
     // rounding)
@@ -871,10 +873,13 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
 #pragma unroll
         for (uint32_t j = 0; j < NIT; ++j) {
             if (j + 1 < NIT || (uint32_t)tid + j * BLOCK < NE) {
-                const float4 v = va[j];
-                sA[slot[j]] = make_float4(v.x, v.y, v.z, luma_fma(v.x, v.y, v.z));
-                sB[slot[j]] = vb[j];
+                // the variance first, then the luminance into the loaded value's own alpha
+                // register (a fresh float4 would copy r, g, b: three moves per entry)
+                float4& v = va[j];
                 sV[slot[j]] = v.w * v.w;
+                v.w = luma_fma(v.x, v.y, v.z);
+                sA[slot[j]] = v;
+                sB[slot[j]] = vb[j];
             }
         }
     }
