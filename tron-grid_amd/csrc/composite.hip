@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 namespace svgf {
 namespace {
@@ -92,6 +93,24 @@ __device__ __forceinline__ f4 load_px(__amdgpu_buffer_rsrc_t r, uint32_t off_row
     return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, off_row + x * 16, 0, 0));
 }
 
+// one pixel's raw words (binary16: 8 bytes, binary32: 16 bytes) and their rgba as float
+template <bool H>
+__device__ __forceinline__ std::conditional_t<H, u2, u4> load_raw(__amdgpu_buffer_rsrc_t r, uint32_t off_row, uint32_t x)
+{
+    if constexpr (H) return __builtin_amdgcn_raw_buffer_load_b64(r, off_row + x * 8, 0, 0);
+    else return __builtin_amdgcn_raw_buffer_load_b128(r, off_row + x * 16, 0, 0);
+}
+template <bool H>
+__device__ __forceinline__ f4 raw_f4(std::conditional_t<H, u2, u4> v)
+{
+    if constexpr (H) {
+        const h4 q = __builtin_bit_cast(h4, v);
+        return f4{(float)q.x, (float)q.y, (float)q.z, (float)q.w};
+    } else {
+        return __builtin_bit_cast(f4, v);
+    }
+}
+
 __device__ __forceinline__ void store_px(__amdgpu_buffer_rsrc_t r, uint32_t off, f4 v, bool half)
 {
     if (half) {
@@ -128,10 +147,114 @@ __device__ __forceinline__ bool reproject(const float* m, uint32_t w, uint32_t h
     return true;
 }
 
+// Planes of a frame that may be RGBA16F (every one but the reflection's kind / hit position).
+constexpr uint32_t HALF_PLANES = ((1u << P_COUNT) - 1u) & ~(1u << P_REFL);
+// Plane formats of a frame as the block body sees them: every colour plane RGBA16F, every one
+// RGBA32F (both compile-time, so all independent loads issue before the first conversion), or
+// mixed (per plane at run time: each load then sits in a branch of its own and waits there).
+enum Formats : int { FM_HALF = 0, FM_FLOAT = 1, FM_MIXED = 2 };
+
+template <int FM>
+__device__ __forceinline__ bool half_of(uint32_t mask, int k)
+{
+    if constexpr (FM == FM_HALF) return k != P_REFL;
+    else if constexpr (FM == FM_FLOAT) return false;
+    else return (mask >> k) & 1u;
+}
+
+// The composite of a block's pixels: the direct loads of all four pixels of the thread first,
+// then the reflection loads that depend on them (the colour where a reflection has one, the
+// history pixel its hit reprojects to or the hit's base colour), then the arithmetic.
+template <int FM>
+__device__ __forceinline__ void composite_pixels(const CompDesc& f, const __amdgpu_buffer_rsrc_t* r, uint32_t p0,
+                                                 float ambient)
+{
+#pragma clang fp contract(off)
+    const bool hh = half_of<FM>(f.half, P_HIST), he = half_of<FM>(f.half, P_EMIS), hf = half_of<FM>(f.half, P_FRES);
+    const bool hc = half_of<FM>(f.half, P_RCOL), hb = half_of<FM>(f.half, P_RBASE), ho = half_of<FM>(f.half, P_OUT);
+    const uint32_t npx = f.w * f.h;
+    uint32_t xs[COMP_PX_PER_THREAD], ys[COMP_PX_PER_THREAD];
+    f4 hist[COMP_PX_PER_THREAD], emis[COMP_PX_PER_THREAD], fres[COMP_PX_PER_THREAD], refl[COMP_PX_PER_THREAD];
+#pragma unroll
+    for (int j = 0; j < COMP_PX_PER_THREAD; ++j) {
+        const uint32_t p = min(p0 + j * COMP_BLOCK, npx - 1);  // clamped tail lanes load a valid pixel
+        ys[j] = p / f.w;
+        xs[j] = p - ys[j] * f.w;
+        hist[j] = load_px(r[P_HIST], ys[j] * f.pitch[P_HIST], xs[j], hh);
+        emis[j] = load_px(r[P_EMIS], ys[j] * f.pitch[P_EMIS], xs[j], he);
+        fres[j] = load_px(r[P_FRES], ys[j] * f.pitch[P_FRES], xs[j], hf);
+        refl[j] = load_px(r[P_REFL], ys[j] * f.pitch[P_REFL], xs[j], false);
+    }
+    f4 col[COMP_PX_PER_THREAD], ind[COMP_PX_PER_THREAD];
+    bool hit[COMP_PX_PER_THREAD];
+    if constexpr (FM == FM_MIXED) {
+#pragma unroll
+        for (int j = 0; j < COMP_PX_PER_THREAD; ++j) {
+            const float kind = refl[j].w;
+            col[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            ind[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            hit[j] = false;
+            if (kind == 1.0f || kind == 2.0f) col[j] = load_px(r[P_RCOL], ys[j] * f.pitch[P_RCOL], xs[j], hc);
+            if (kind == 2.0f) {
+                uint32_t px, py;
+                hit[j] = reproject(f.m, f.w, f.h, refl[j].x, refl[j].y, refl[j].z, px, py);
+                ind[j] = hit[j] ? load_px(r[P_HIST], py * f.pitch[P_HIST], px, hh)
+                                : load_px(r[P_RBASE], ys[j] * f.pitch[P_RBASE], xs[j], hb);
+            }
+        }
+    } else {
+        // the raw words are loaded in the branches and converted after all of them are issued
+        // (a conversion inside a branch would wait there for its load)
+        constexpr bool H = FM == FM_HALF;
+        using Raw = std::conditional_t<H, u2, u4>;
+        // every pixel's reprojection first: computed after one wait for the direct loads, where
+        // a gather issued in between would have to complete too (loads retire in order)
+        uint32_t gx[COMP_PX_PER_THREAD], gy[COMP_PX_PER_THREAD];
+#pragma unroll
+        for (int j = 0; j < COMP_PX_PER_THREAD; ++j) {
+            gx[j] = xs[j];
+            gy[j] = ys[j];
+            hit[j] = refl[j].w == 2.0f && reproject(f.m, f.w, f.h, refl[j].x, refl[j].y, refl[j].z, gx[j], gy[j]);
+        }
+        // No branch: a load a pixel does not need takes an offset past its plane, which the
+        // buffer resource answers with 0 and no memory access (the ABI keeps planes under
+        // 2 GiB). The loads of all four pixels are then in flight before the first conversion.
+        constexpr uint32_t OOB = 0x80000000u;
+        Raw rcol[COMP_PX_PER_THREAD], rhist[COMP_PX_PER_THREAD], rbase[COMP_PX_PER_THREAD];
+#pragma unroll
+        for (int j = 0; j < COMP_PX_PER_THREAD; ++j) {
+            const float kind = refl[j].w;
+            const bool has_col = kind == 1.0f || kind == 2.0f, miss = kind == 2.0f && !hit[j];
+            rcol[j] = load_raw<H>(r[P_RCOL], has_col ? ys[j] * f.pitch[P_RCOL] : OOB, has_col ? xs[j] : 0u);
+            rhist[j] = load_raw<H>(r[P_HIST], hit[j] ? gy[j] * f.pitch[P_HIST] : OOB, hit[j] ? gx[j] : 0u);
+            rbase[j] = load_raw<H>(r[P_RBASE], miss ? ys[j] * f.pitch[P_RBASE] : OOB, miss ? xs[j] : 0u);
+        }
+#pragma unroll
+        for (int j = 0; j < COMP_PX_PER_THREAD; ++j) {
+            col[j] = raw_f4<H>(rcol[j]);
+            ind[j] = hit[j] ? raw_f4<H>(rhist[j]) : raw_f4<H>(rbase[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < COMP_PX_PER_THREAD; ++j) {
+        if (p0 + j * COMP_BLOCK >= npx) break;
+        const float kind = refl[j].w;
+        f4 rc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (kind == 2.0f) {
+            const f4 in = hit[j] ? ind[j] : f4{ind[j].x * ambient, ind[j].y * ambient, ind[j].z * ambient, 0.0f};
+            rc = f4{col[j].x + in.x, col[j].y + in.y, col[j].z + in.z, 0.0f};
+        } else if (kind == 1.0f) {
+            rc = col[j];
+        }
+        const f4 o = {(hist[j].x + emis[j].x) + rc.x * fres[j].x, (hist[j].y + emis[j].y) + rc.y * fres[j].y,
+                      (hist[j].z + emis[j].z) + rc.z * fres[j].z, 1.0f};
+        store_px(r[P_OUT], ys[j] * f.pitch[P_OUT] + xs[j] * (ho ? 8u : 16u), o, ho);
+    }
+}
+
 __device__ __forceinline__ void composite_block(const uint32_t* starts, const CompDesc* frames, uint32_t n,
                                                 float ambient)
 {
-#pragma clang fp contract(off)
     const uint32_t b = blockIdx.x;
     if (b >= starts[n]) return;
     const uint32_t fi = frame_of(starts, n, b);
@@ -145,46 +268,11 @@ __device__ __forceinline__ void composite_block(const uint32_t* starts, const Co
     __amdgpu_buffer_rsrc_t r[P_COUNT];
 #pragma unroll
     for (int k = 0; k < P_COUNT; ++k) r[k] = rsrc(f.plane[k], f.pitch[k] * f.h);
-    const bool hh = f.half & (1u << P_HIST), he = f.half & (1u << P_EMIS), hf = f.half & (1u << P_FRES);
-    const bool hc = f.half & (1u << P_RCOL), hb = f.half & (1u << P_RBASE), ho = f.half & (1u << P_OUT);
-    const uint32_t npx = f.w * f.h;
     const uint32_t p0 = (b - starts[fi]) * COMP_PX_PER_BLOCK + threadIdx.x;
-
-    uint32_t xs[COMP_PX_PER_THREAD], ys[COMP_PX_PER_THREAD];
-    f4 hist[COMP_PX_PER_THREAD], emis[COMP_PX_PER_THREAD], fres[COMP_PX_PER_THREAD], refl[COMP_PX_PER_THREAD];
-#pragma unroll
-    for (int j = 0; j < COMP_PX_PER_THREAD; ++j) {
-        const uint32_t p = min(p0 + j * COMP_BLOCK, npx - 1);  // clamped tail lanes load a valid pixel
-        ys[j] = p / f.w;
-        xs[j] = p - ys[j] * f.w;
-        hist[j] = load_px(r[P_HIST], ys[j] * f.pitch[P_HIST], xs[j], hh);
-        emis[j] = load_px(r[P_EMIS], ys[j] * f.pitch[P_EMIS], xs[j], he);
-        fres[j] = load_px(r[P_FRES], ys[j] * f.pitch[P_FRES], xs[j], hf);
-        refl[j] = load_px(r[P_REFL], ys[j] * f.pitch[P_REFL], xs[j], false);
-    }
-#pragma unroll
-    for (int j = 0; j < COMP_PX_PER_THREAD; ++j) {
-        if (p0 + j * COMP_BLOCK >= npx) break;
-        const float kind = refl[j].w;
-        f4 rc = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (kind == 2.0f) {
-            const f4 col = load_px(r[P_RCOL], ys[j] * f.pitch[P_RCOL], xs[j], hc);
-            uint32_t px, py;
-            f4 ind;
-            if (reproject(f.m, f.w, f.h, refl[j].x, refl[j].y, refl[j].z, px, py)) {
-                ind = load_px(r[P_HIST], py * f.pitch[P_HIST], px, hh);
-            } else {
-                const f4 base = load_px(r[P_RBASE], ys[j] * f.pitch[P_RBASE], xs[j], hb);
-                ind = f4{base.x * ambient, base.y * ambient, base.z * ambient, 0.0f};
-            }
-            rc = f4{col.x + ind.x, col.y + ind.y, col.z + ind.z, 0.0f};
-        } else if (kind == 1.0f) {
-            rc = load_px(r[P_RCOL], ys[j] * f.pitch[P_RCOL], xs[j], hc);
-        }
-        const f4 o = {(hist[j].x + emis[j].x) + rc.x * fres[j].x, (hist[j].y + emis[j].y) + rc.y * fres[j].y,
-                      (hist[j].z + emis[j].z) + rc.z * fres[j].z, 1.0f};
-        store_px(r[P_OUT], ys[j] * f.pitch[P_OUT] + xs[j] * (ho ? 8u : 16u), o, ho);
-    }
+    const uint32_t mask = f.half & HALF_PLANES;
+    if (mask == HALF_PLANES) composite_pixels<FM_HALF>(f, r, p0, ambient);
+    else if (mask == 0u) composite_pixels<FM_FLOAT>(f, r, p0, ambient);
+    else composite_pixels<FM_MIXED>(f, r, p0, ambient);
 }
 
 __global__ __launch_bounds__(COMP_BLOCK) void k_composite(CompArgs a)

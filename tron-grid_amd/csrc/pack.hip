@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 namespace svgf {
 namespace {
@@ -106,36 +107,110 @@ __device__ __forceinline__ void unpack_octahedral(uint32_t packed, float& x, flo
     z = nz * inv;
 }
 
-__device__ __forceinline__ void pack_block(const uint32_t* starts, const PackDesc* frames, uint32_t n)
+// Planes of a frame that may be RGBA16F: the three lighting planes and the radiance.
+constexpr uint32_t PK_HALF_PLANES = (1u << Q_IND) | (1u << Q_DIF) | (1u << Q_SPC) | (1u << Q_RAD);
+// Plane formats as the block body sees them: all four RGBA16F, all RGBA32F (compile-time, so
+// the lighting loads of all four pixels issue before the first conversion), or mixed (per
+// plane at run time: each lighting load then waits in a branch of its own).
+enum Formats : int { FM_HALF = 0, FM_FLOAT = 1, FM_MIXED = 2 };
+
+template <int FM>
+__device__ __forceinline__ bool half_of(uint32_t mask, int k)
+{
+    if constexpr (FM == FM_HALF) return true;
+    else if constexpr (FM == FM_FLOAT) return false;
+    else return (mask >> k) & 1u;
+}
+
+// one pixel's raw words (binary16: 8 bytes, binary32: 16 bytes) and their rgba as float
+template <bool H>
+__device__ __forceinline__ std::conditional_t<H, u2, u4> load_raw(__amdgpu_buffer_rsrc_t r, uint32_t off_row, uint32_t x)
+{
+    if constexpr (H) return __builtin_amdgcn_raw_buffer_load_b64(r, off_row + x * 8, 0, 0);
+    else return __builtin_amdgcn_raw_buffer_load_b128(r, off_row + x * 16, 0, 0);
+}
+template <bool H>
+__device__ __forceinline__ f4 raw_f4(std::conditional_t<H, u2, u4> v)
+{
+    if constexpr (H) {
+        const h4 q = __builtin_bit_cast(h4, v);
+        return f4{(float)q.x, (float)q.y, (float)q.z, (float)q.w};
+    } else {
+        return __builtin_bit_cast(f4, v);
+    }
+}
+
+// The pack of a block's pixels: position and normal of all four pixels of the thread, then the
+// lighting terms of the covered ones, then the arithmetic and the stores.
+template <int FM>
+__device__ __forceinline__ void pack_pixels(const PackDesc& f, const __amdgpu_buffer_rsrc_t* r, uint32_t p0)
 {
 #pragma clang fp contract(off)
-    const uint32_t b = blockIdx.x;
-    if (b >= starts[n]) return;
-    const uint32_t fi = frame_of(starts, n, b);
-    PackDesc f;
-    {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(frames + fi);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&f);
-#pragma unroll
-        for (int k = 0; k < (int)(sizeof(PackDesc) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
-    }
-    __amdgpu_buffer_rsrc_t r[Q_COUNT];
-#pragma unroll
-    for (int k = 0; k < Q_COUNT; ++k) r[k] = rsrc(f.plane[k], f.pitch[k] * f.h);
-    const bool hi = f.half & (1u << Q_IND), hd = f.half & (1u << Q_DIF), hs = f.half & (1u << Q_SPC);
-    const bool hr = f.half & (1u << Q_RAD);
+    const bool hi = half_of<FM>(f.half, Q_IND), hd = half_of<FM>(f.half, Q_DIF), hs = half_of<FM>(f.half, Q_SPC);
+    const bool hr = half_of<FM>(f.half, Q_RAD);
     const uint32_t npx = f.w * f.h;
-    const uint32_t p0 = (b - starts[fi]) * PK_PX_PER_BLOCK + threadIdx.x;
 
     uint32_t xs[PK_PX_PER_THREAD], ys[PK_PX_PER_THREAD], oct[PK_PX_PER_THREAD];
     f4 pos[PK_PX_PER_THREAD];
+    // the four pixels' coordinates first, then their loads back to back (otherwise the
+    // optimiser reads one pixel's coverage while the next pixel's address is still being
+    // computed, and that read waits for the loads already issued)
 #pragma unroll
     for (int j = 0; j < PK_PX_PER_THREAD; ++j) {
         const uint32_t p = min(p0 + j * PK_BLOCK, npx - 1);  // clamped tail lanes load a valid pixel
         ys[j] = p / f.w;
         xs[j] = p - ys[j] * f.w;
+    }
+#pragma unroll
+    for (int j = 0; j < PK_PX_PER_THREAD; ++j) asm volatile("" ::"v"(ys[j]), "v"(xs[j]));
+#pragma unroll
+    for (int j = 0; j < PK_PX_PER_THREAD; ++j) {
         pos[j] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r[Q_POS], ys[j] * f.pitch[Q_POS] + xs[j] * 16, 0, 0));
         oct[j] = __builtin_amdgcn_raw_buffer_load_b32(r[Q_NRM], ys[j] * f.pitch[Q_NRM] + xs[j] * 4, 0, 0);
+    }
+    f4 ind[PK_PX_PER_THREAD], dif[PK_PX_PER_THREAD], spc[PK_PX_PER_THREAD];
+    if constexpr (FM == FM_MIXED) {
+#pragma unroll
+        for (int j = 0; j < PK_PX_PER_THREAD; ++j) {
+            ind[j] = dif[j] = spc[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+            if (pos[j].w > 0.0f) {
+                ind[j] = load4(r[Q_IND], ys[j] * f.pitch[Q_IND], xs[j], hi);
+                dif[j] = load4(r[Q_DIF], ys[j] * f.pitch[Q_DIF], xs[j], hd);
+                spc[j] = load4(r[Q_SPC], ys[j] * f.pitch[Q_SPC], xs[j], hs);
+            }
+        }
+    } else {
+        // No branch: an uncovered pixel's lighting loads take an offset past the plane, which
+        // the buffer resource answers with 0 and no memory access (the ABI keeps planes under
+        // 2 GiB). Straight-line loads of all four pixels are all in flight before the first
+        // conversion, where a load in a branch would be waited for at the branch's end.
+        constexpr bool H = FM == FM_HALF;
+        using Raw = std::conditional_t<H, u2, u4>;
+        constexpr uint32_t OOB = 0x80000000u;
+        // every pixel's coverage is read before the first lighting load is issued: a load
+        // issued in between would have to complete too (loads retire in order)
+        uint32_t cy[PK_PX_PER_THREAD], cx[PK_PX_PER_THREAD];
+#pragma unroll
+        for (int j = 0; j < PK_PX_PER_THREAD; ++j) {
+            const bool cov = pos[j].w > 0.0f;
+            cy[j] = cov ? ys[j] : OOB;  // row index: the planes' row offsets exceed 2^31 past it
+            cx[j] = cov ? xs[j] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < PK_PX_PER_THREAD; ++j) asm volatile("" ::"v"(cy[j]), "v"(cx[j]));
+        Raw ri[PK_PX_PER_THREAD], rd[PK_PX_PER_THREAD], rs[PK_PX_PER_THREAD];
+#pragma unroll
+        for (int j = 0; j < PK_PX_PER_THREAD; ++j) {
+            ri[j] = load_raw<H>(r[Q_IND], cy[j] == OOB ? OOB : cy[j] * f.pitch[Q_IND], cx[j]);
+            rd[j] = load_raw<H>(r[Q_DIF], cy[j] == OOB ? OOB : cy[j] * f.pitch[Q_DIF], cx[j]);
+            rs[j] = load_raw<H>(r[Q_SPC], cy[j] == OOB ? OOB : cy[j] * f.pitch[Q_SPC], cx[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < PK_PX_PER_THREAD; ++j) {
+            ind[j] = raw_f4<H>(ri[j]);
+            dif[j] = raw_f4<H>(rd[j]);
+            spc[j] = raw_f4<H>(rs[j]);
+        }
     }
 #pragma unroll
     for (int j = 0; j < PK_PX_PER_THREAD; ++j) {
@@ -144,11 +219,8 @@ __device__ __forceinline__ void pack_block(const uint32_t* starts, const PackDes
         f4 rad = {0.0f, 0.0f, 0.0f, 0.0f}, g = {0.0f, 0.0f, 0.0f, 0.0f};
         f2 mv = {NO_MOTION, NO_MOTION};
         if (pos[j].w > 0.0f) {
-            const f4 ind = load4(r[Q_IND], y * f.pitch[Q_IND], x, hi);
-            const f4 dif = load4(r[Q_DIF], y * f.pitch[Q_DIF], x, hd);
-            const f4 spc = load4(r[Q_SPC], y * f.pitch[Q_SPC], x, hs);
-            rad = f4{(ind.x + dif.x) * dif.w + spc.x, (ind.y + dif.y) * dif.w + spc.y,
-                     (ind.z + dif.z) * dif.w + spc.z, 1.0f};
+            rad = f4{(ind[j].x + dif[j].x) * dif[j].w + spc[j].x, (ind[j].y + dif[j].y) * dif[j].w + spc[j].y,
+                     (ind[j].z + dif[j].z) * dif[j].w + spc[j].z, 1.0f};
             float ox, oy, oz;
             unpack_octahedral(oct[j], ox, oy, oz);
             g.x = ox;
@@ -175,6 +247,28 @@ __device__ __forceinline__ void pack_block(const uint32_t* starts, const PackDes
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, g), r[Q_G], y * f.pitch[Q_G] + x * 16, 0, 0);
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, mv), r[Q_MV], y * f.pitch[Q_MV] + x * 8, 0, 0);
     }
+}
+
+__device__ __forceinline__ void pack_block(const uint32_t* starts, const PackDesc* frames, uint32_t n)
+{
+    const uint32_t b = blockIdx.x;
+    if (b >= starts[n]) return;
+    const uint32_t fi = frame_of(starts, n, b);
+    PackDesc f;
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(frames + fi);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&f);
+#pragma unroll
+        for (int k = 0; k < (int)(sizeof(PackDesc) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+    }
+    __amdgpu_buffer_rsrc_t r[Q_COUNT];
+#pragma unroll
+    for (int k = 0; k < Q_COUNT; ++k) r[k] = rsrc(f.plane[k], f.pitch[k] * f.h);
+    const uint32_t p0 = (b - starts[fi]) * PK_PX_PER_BLOCK + threadIdx.x;
+    const uint32_t mask = f.half & PK_HALF_PLANES;
+    if (mask == PK_HALF_PLANES) pack_pixels<FM_HALF>(f, r, p0);
+    else if (mask == 0u) pack_pixels<FM_FLOAT>(f, r, p0);
+    else pack_pixels<FM_MIXED>(f, r, p0);
 }
 
 __global__ __launch_bounds__(PK_BLOCK) void k_pack(PackArgs a) { pack_block(a.starts, a.frames, a.n); }

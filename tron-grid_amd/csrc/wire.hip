@@ -143,33 +143,40 @@ __global__ __launch_bounds__(WIRE_BLOCK) void k_wire(WireArgs a)
     uint32_t rgb[4] = {0, 0, 0, 0};
     float dep[4] = {1.0f, 1.0f, 1.0f, 1.0f};
     const int cnt = (int)min<uint64_t>(WIRE_PX_PER_THREAD, npx - p0);
+    // The colour and depth of all four pixels are loaded before the first is encoded (one
+    // memory round trip). A pixel past the frame's last (the tail of the last block) lies past
+    // the planes, so the buffer resources answer its loads with 0; it is neither encoded nor
+    // stored.
+    float cr_[WIRE_PX_PER_THREAD], cg_[WIRE_PX_PER_THREAD], cb_[WIRE_PX_PER_THREAD], z_[WIRE_PX_PER_THREAD];
 #pragma unroll
     for (int j = 0; j < WIRE_PX_PER_THREAD; ++j) {
-        if (j < cnt) {
-            float r, g, bl;
-            if constexpr (FP16) {
-                typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-                const uint32_t off = y * (uint32_t)f.colour_pitch + x * 8;
-                const h4 v = __builtin_bit_cast(h4, __builtin_amdgcn_raw_buffer_load_b64(cr, off, 0, 0));
-                r = (float)v.x;
-                g = (float)v.y;
-                bl = (float)v.z;
-            } else {
-                const uint32_t off = y * (uint32_t)f.colour_pitch + x * 16;
-                typedef float f4 __attribute__((ext_vector_type(4)));
-                const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(cr, off, 0, 0));
-                r = v.x;
-                g = v.y;
-                bl = v.z;
-            }
-            const float z = __builtin_bit_cast(
-                float, __builtin_amdgcn_raw_buffer_load_b32(gr, y * (uint32_t)f.gbuffer_pitch + x * 16 + 12, 0, 0));
-            rgb[j] = encode_rgb(sT, r, g, bl, a.exposure);
-            dep[j] = encode_depth(z, a.near_z, a.far_z);
+        if constexpr (FP16) {
+            typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+            const uint32_t off = y * (uint32_t)f.colour_pitch + x * 8;
+            const h4 v = __builtin_bit_cast(h4, __builtin_amdgcn_raw_buffer_load_b64(cr, off, 0, 0));
+            cr_[j] = (float)v.x;
+            cg_[j] = (float)v.y;
+            cb_[j] = (float)v.z;
+        } else {
+            const uint32_t off = y * (uint32_t)f.colour_pitch + x * 16;
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(cr, off, 0, 0));
+            cr_[j] = v.x;
+            cg_[j] = v.y;
+            cb_[j] = v.z;
         }
+        z_[j] = __builtin_bit_cast(
+            float, __builtin_amdgcn_raw_buffer_load_b32(gr, y * (uint32_t)f.gbuffer_pitch + x * 16 + 12, 0, 0));
         if (++x == f.w) {
             x = 0;
             ++y;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < WIRE_PX_PER_THREAD; ++j) {
+        if (j < cnt) {
+            rgb[j] = encode_rgb(sT, cr_[j], cg_[j], cb_[j], a.exposure);
+            dep[j] = encode_depth(z_[j], a.near_z, a.far_z);
         }
     }
     const __amdgpu_buffer_rsrc_t rr = rsrc(f.rgb, npx * 3);
