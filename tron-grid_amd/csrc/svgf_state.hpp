@@ -42,10 +42,11 @@ constexpr uint32_t WIDE_MAX_LEN = WIDE_LEN_MASK;
 
 SVGF_HD inline float f32_from_bits(uint32_t b) { return __builtin_bit_cast(float, b); }
 
-// m = max(biased exponent, 1) of a binary16 bit pattern
+// m = max(biased exponent, 1) of a binary16 bit pattern (a 32-bit field extract: the 16-bit
+// shift and mask take an instruction more on the GPU)
 SVGF_HD inline uint32_t half_scale_exp(uint16_t hb)
 {
-    const uint32_t e = (hb >> 10) & 31u;
+    const uint32_t e = ((uint32_t)hb >> 10) & 31u;
     return e > 1u ? e : 1u;
 }
 
@@ -55,10 +56,11 @@ SVGF_HD inline _Float16 wide_encode_channel(float v, uint32_t& code)
     const _Float16 h = (_Float16)v;
     const uint16_t hb = __builtin_bit_cast(uint16_t, h);
     float q = 0.0f;
-    if (((hb >> 10) & 31u) != 31u) {
+    if ((((uint32_t)hb >> 10) & 31u) != 31u) {
         const float inv_step = f32_from_bits((155u - half_scale_exp(hb)) << 23);  // 2^(28 - m)
-        q = __builtin_rintf((v - (float)h) * inv_step);
-        q = q < -4.0f ? -4.0f : (q > 3.0f ? 3.0f : q);
+        // h is finite here, so v is and q is: min / max clamp it as the comparisons would
+        // (one v_med3_f32 on the GPU)
+        q = __builtin_fminf(__builtin_fmaxf(__builtin_rintf((v - (float)h) * inv_step), -4.0f), 3.0f);
     }
     code = (uint32_t)(int)q & 7u;
     return h;
