@@ -51,9 +51,13 @@ def stage_bytes(storage_bytes4, storage_bytes2, iterations, g_copy):
     """storage_bytes2: bytes of a moments pixel (8 for the wide FP16 state's four binary16 words);
     g_copy: the temporal stage writes a copy of G for the stream's next frame."""
     s4, s2, w4 = storage_bytes4, storage_bytes2, 16
-    temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (w4 + s2 + 1 + 8) + (16 if g_copy else 0)
+    # temporal: R, G, motion, G_prev, history, moments read; W0, moments, history length written
+    temporal = (s4 + 16 + 8 + 16 + s4 + s2) + (w4 + s2 + 1) + (16 if g_copy else 0)
+    # a-trous k: W_k, G (and from k = 1 the depth gradient) read, W_k+1 (last: the output)
+    # written; iteration 0 also reads the history length and writes the colour history and,
+    # when strided iterations follow, the depth gradient
     atrous = [(w4 + 16) + (0 if k == 0 else 8) + (s4 if k == iterations - 1 else w4)
-              + ((1 + s4) if k == 0 else 0) for k in range(iterations)]
+              + ((1 + s4 + (8 if iterations > 1 else 0)) if k == 0 else 0) for k in range(iterations)]
     return [temporal] + atrous
 
 

@@ -159,7 +159,6 @@ struct F16S {
 
 // float4 G-buffer, float2 motion (FP32 in both storage forms)
 __device__ __forceinline__ float4 ldg4(const Buf& b, int x, int y) { return ld_f4(b, y * b.pitch + x * 16); }
-__device__ __forceinline__ float ldgz(const Buf& b, int x, int y) { return ld_f1(b, y * b.pitch + x * 16 + 12); }
 __device__ __forceinline__ void stg4(const Buf& b, int x, int y, float4 v) { st_f4(b, y * b.pitch + x * 16, v); }
 __device__ __forceinline__ float2 ldg2(const Buf& b, int x, int y) { return ld_f2(b, y * b.pitch + x * 8); }
 
@@ -290,7 +289,6 @@ __host__ __device__ constexpr uint32_t atrous_lds_bytes(bool first, int nr, bool
     return ((entries * 36u + list + 15u) & ~15u) + 16u;
 }
 constexpr uint32_t REF_ATROUS_LDS = LREG * LREG * 16;
-constexpr uint32_t TEMPORAL_LDS = (LT + 2) * (LT + 2) * 4;
 constexpr uint32_t cmax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 
 // A kernel that runs one stage gives its tile function arrays of its own (OwnLds). The
@@ -306,7 +304,7 @@ struct OwnLds {
 };
 constexpr uint32_t PERSIST_LDS_CAN =
     cmax(cmax(atrous_lds_bytes(true, 1, true), atrous_lds_bytes(false, 2, false)),
-         cmax(atrous_lds_bytes(false, 1, false), TEMPORAL_LDS));
+         atrous_lds_bytes(false, 1, false));
 __shared__ __align__(16) char g_persist_lds_can[PERSIST_LDS_CAN];
 __shared__ __align__(16) char g_persist_lds_ref[REF_ATROUS_LDS];
 template <bool CANON>
@@ -432,22 +430,15 @@ __device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(
 // variance)) into wavelet_a, the history length (uint8) and the moments; the batched form
 // also keeps G for the next frame. Standard deviations rather than variances are stored so
 // that heavy-tailed radiance stays inside the FP16 range.
-// Pixel (x, y) of the frame, with its G, radiance R, motion m (unused without history) and the
-// depths of its four clamped neighbours.
+// Pixel (x, y) of the frame, with its G, radiance R and motion m (unused without history).
 template <class S, int SF>
 __device__ __forceinline__ void can_temporal_pixel(const FrameDesc& f, const KParams& p, int x, int y, float4 G,
-                                                   float4 R, float2 m, float zl, float zr, float zu, float zd)
+                                                   float4 R, float2 m)
 {
     constexpr bool COMP = SF != SF_RAW, WIDE = SF == SF_WIDE;
     static_assert(!WIDE || S::B4 == 8, "the wide state is a binary16 form");
     float l = luma_fma(R.x, R.y, R.z);
     if (f.gsave.ptr) stg4(bind(f.gsave, f.h), x, y, G);
-    {
-        // depth gradient (central differences of clamped neighbours, halved), kept for the
-        // a-trous stages of this frame
-        const Buf gr = bind(f.grad, f.h);
-        st_f2(gr, y * gr.pitch + x * 8, make_float2(0.5f * (zr - zl), 0.5f * (zd - zu)));
-    }
 
     float sumb = 0.0f, h0 = 0.0f, h1 = 0.0f, h2 = 0.0f, hm1 = 0.0f, hm2 = 0.0f;
     // Compensated form: per-tap (weight, mean, second moment), weight 0 for unused taps so
@@ -578,47 +569,17 @@ __device__ __forceinline__ void can_temporal_tile(const FrameDesc& f, const KPar
     const int tid = lane_tid();
     const int ti = tid & (LT - 1), tj = tid >> 4;
     const int x = T.i0 + ti, y = T.j0 + tj;
-    const int max_x = f.w - 1, max_y = f.h - 1;
-    const Buf gb = bind(f.g, f.h);
-    // Depths of the tile and a one-pixel border, clamped into the frame, shared through LDS for
-    // the depth gradient: one G load per pixel instead of a G load and four depth loads. Lanes
-    // past the frame edge load the clamped pixel, which is what their neighbours' clamped
-    // lookups read.
-    constexpr int ZW = LT + 2;
-    float* const sZ = reinterpret_cast<float*>(L::template get<TEMPORAL_LDS>());
-    const int cx = min(x, max_x), cy = min(y, max_y);
-    const float4 G = ldg4(gb, cx, cy);
-    // border: the rows above and below (ZW entries each), then the columns left and right
-    const bool border = tid < 4 * LT + 4;
-    int bslot = 0;
-    float zb = 0.0f;
-    if (border) {
-        int bx, by;
-        if (tid < 2 * ZW) {
-            bx = tid % ZW - 1;
-            by = tid < ZW ? -1 : LT;
-        } else {
-            const int c = tid - 2 * ZW;
-            bx = (c & 1) ? LT : -1;
-            by = c >> 1;
-        }
-        bslot = (by + 1) * ZW + bx + 1;
-        zb = ldgz(gb, clampi(T.i0 + bx, 0, max_x), clampi(T.j0 + by, 0, max_y));
+    if (x < f.w && y < f.h) {
+        // G, radiance and motion in one memory round trip, the reprojection gathers in a
+        // second. The depth gradient the strided a-trous iterations use is stored by iteration
+        // 0, which has the neighbours' depths staged (the same floats).
+        const float4 G = ldg4(bind(f.g, f.h), x, y);
+        const float4 R = S::ld4(bind(f.rad, f.h), x, y);
+        const float2 m = f.history_valid ? ldg2(bind(f.mv, f.h), x, y) : make_float2(0.0f, 0.0f);
+        can_temporal_pixel<S, SF>(f, p, x, y, G, R, m);
     }
-    // The pixel's radiance and motion are loaded with G, before the barrier (whose fence keeps
-    // later loads behind it): the border depths, G, R and the motion then take one memory round
-    // trip together, and the reprojection gathers a second.
-    const float4 R = S::ld4(bind(f.rad, f.h), cx, cy);
-    const float2 m = f.history_valid ? ldg2(bind(f.mv, f.h), cx, cy) : make_float2(0.0f, 0.0f);
-    sZ[(tj + 1) * ZW + ti + 1] = G.w;
-    if (border) sZ[bslot] = zb;
-    lds_writes_done();
+    // the tile functions of the persistent kernel's ticket loop all end at a barrier
     __syncthreads();
-    if (x <= max_x && y <= max_y) {
-        const int c = (tj + 1) * ZW + ti + 1;
-        can_temporal_pixel<S, SF>(f, p, x, y, G, R, m, sZ[c - 1], sZ[c + 1], sZ[c - ZW], sZ[c + ZW]);
-    }
-    __syncthreads();  // the next tile of a grid-stride loop reuses sZ
 }
 
 // Spatial variance of a young pixel (history length n < 4): the edge-aware variance of the
@@ -808,8 +769,8 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     constexpr int NQ = FIRST ? (QW * QH + BLOCK - 1) / BLOCK : 0;
     uint32_t nq[NQ > 0 ? NQ : 1];
     // The thread's lattice points: (x, y0) and, in blocked tiles, (x, y1) with y1 = y0 + s.
-    // Their depth gradients (strided iterations: the temporal stage's plane; iteration 0 takes
-    // them from the staged depths) or history lengths (iteration 0) are loaded at coordinates
+    // Their depth gradients (strided iterations: the plane iteration 0 stores; iteration 0
+    // computes them from the staged depths) or history lengths (iteration 0) are loaded at coordinates
     // clamped into the frame (the values of in-frame points). EARLY: together with the staged
     // region, whose loads all complete before its first LDS write, one memory round trip for
     // the whole tile. Otherwise after the barrier: at strides of 8 and more every lane's
@@ -947,6 +908,8 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
             const float4 C = sA[c0];
             const float2 g = FIRST ? make_float2(0.5f * (sB[c0 + 1].w - sB[c0 - 1].w), 0.5f * (sB[c0 + RW].w - sB[c0 - RW].w))
                                    : g0;
+            // iteration 0 keeps the gradient for the strided iterations
+            if (FIRST && p.iterations > 1) st_f2(gr, y * gr.pitch + x * 8, g);
             const uint32_t n = n0;
             const Centre c = centre_of(p, C, sB[c0], prefilter(c0), g.x, g.y, s);
             Sums<!LAST> a;
@@ -972,6 +935,11 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
                 // central differences, the same floats)
                 g0 = make_float2(0.5f * (sB[c0 + 1].w - sB[c0 - 1].w), 0.5f * (sB[c1].w - sB[c0 - RW].w));
                 g1 = make_float2(0.5f * (sB[c1 + 1].w - sB[c1 - 1].w), 0.5f * (sB[c1 + RW].w - sB[c0].w));
+                // kept for the strided iterations
+                if (p.iterations > 1) {
+                    st_f2(gr, y0 * gr.pitch + x * 8, g0);
+                    if (y1 <= max_y) st_f2(gr, y1 * gr.pitch + x * 8, g1);
+                }
             }
             const Centre e0 = centre_of(p, C0, sB[c0], prefilter(c0), g0.x, g0.y, s);
             const Centre e1 = centre_of(p, C1, sB[c1], prefilter(c1), g1.x, g1.y, s);
