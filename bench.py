@@ -330,6 +330,61 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
                 t_enc += ev[2].elapsed_time(ev[3])
                 t_rb += ev[3].elapsed_time(ev[4])
     t_den, t_cmp, t_enc, t_rb = t_den / ticks, t_cmp / ticks, t_enc / ticks, t_rb / ticks
+    # Pipelined ticks: tick k encodes into wire buffer k % 2 (the frame parity); once tick k + 1's
+    # denoise and composite are enqueued, the host waits for tick k's encode and pushes its buffer
+    # on a second stream, and tick k + 2's encode waits for that push, so one tick's readback runs
+    # under the next tick's denoise and composite. The push is enqueued with its input already
+    # complete: a device-to-host copy ordered behind another stream's event is run by the HIP
+    # runtime as a copy kernel instead of on the DMA engine, and its writes across PCIe then slow
+    # the memory-bound temporal stage beside it about 7x.
+    packed2 = [packed, torch.empty_like(packed)]
+    wfp = []
+    for f in range(2):
+        warr = (S.WireFrame * agents)()
+        for i in range(agents):
+            ctypes.memmove(ctypes.byref(warr[i]), ctypes.byref(wfs[f][i]), ctypes.sizeof(S.WireFrame))
+            warr[i].rgb = packed2[f].data_ptr() + rgb_off[i]
+            warr[i].depth = packed2[f].data_ptr() + dep_off[i]
+        wfp.append(warr)
+    cstream = torch.cuda.Stream(device=dev)
+    csh = ctypes.c_void_p(cstream.cuda_stream)
+    e_enc = [torch.cuda.Event() for _ in range(2)]
+    e_rb = [torch.cuda.Event() for _ in range(2)]
+    p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+
+    def push(b):
+        e_enc[b].synchronize()
+        S.check(L.svgf_readback_push(rb.handle, ctypes.c_void_p(packed2[b].data_ptr()), off, csh, ctypes.byref(seq)),
+                "svgf_readback_push")
+        e_rb[b].record(cstream)
+
+    n = warmup + ticks
+    # the denoise of each timed tick between events: it runs at its sequential speed when the
+    # readback beside it stays on the DMA engine
+    d_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(ticks)]
+    for k in range(n):
+        b = (k + n) % 2  # the frame parity continues from the ticks above
+        if k == warmup:
+            p0.record(stream)
+        if k >= warmup:
+            d_ev[k - warmup][0].record(stream)
+        S.check(L.svgf_denoise_batch(ctx.handle, reqs[b], agents, sh), "svgf_denoise_batch")
+        if k >= warmup:
+            d_ev[k - warmup][1].record(stream)
+        S.check(L.svgf_composite_run(comp.handle, carr, agents, ctypes.byref(cp), sh), "svgf_composite_run")
+        if k >= 1:
+            push(1 - b)
+        if k >= 2:
+            stream.wait_event(e_rb[b])
+        S.check(L.svgf_wire_encode(wire.handle, wfp[b], agents, ctypes.byref(wp), sh), "svgf_wire_encode")
+        e_enc[b].record(stream)
+    push((2 * n - 1) % 2)
+    stream.wait_event(e_rb[(2 * n - 1) % 2])
+    p1.record(stream)
+    p1.synchronize()
+    t_pipe = p0.elapsed_time(p1) / ticks
+    t_pipe_den = sum(e0.elapsed_time(e1) for e0, e1 in d_ev) / ticks
     out = {
         "what": f"{agents} agent streams, eye sizes uniform over 8x8..256x256 (docs/AI_INTERFACE.md:354-360): "
                 "batched FP16 canonical SVGF, consumer composite over the denoised output in one launch, wire encode (fitted ACES + sRGB RGB888, depth in [0, 1]) in one launch, "
@@ -346,6 +401,11 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
                             "frac": round(npx * WIRE_BYTES_PX / t_enc / 1e6 / HBM_PEAK_GBPS, 4)},
         "readback_ms": round(t_rb, 4),
         "readback_GBps": round(off / t_rb / 1e6, 2),
+        "pipelined_tick_ms": round(t_pipe, 4),
+        "pipelined_denoise_ms": round(t_pipe_den, 4),
+        "pipelined_what": "ticks back to back: wire buffers alternate, each tick's readback on a second stream "
+                          "(DMA engine) under the next tick's denoise and composite (first timed tick's start to the "
+                          "last push's end / ticks)",
     }
     if cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
@@ -565,11 +625,11 @@ def main():
                     help="HBM bytes per a-trous launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE (reported as is)")
     ap.add_argument("--no-ref720", dest="ref720", action="store_false",
                     help="skip the REFERENCE-mode 1280x720 timing (the reference's published case)")
-    ap.add_argument("--report", choices=["precision", "copy", "composite", "pack", "valu"], default=None,
+    ap.add_argument("--report", choices=["precision", "copy", "composite", "pack", "valu", "agents"], default=None,
                     help="precision: error of the GPU forms vs the FP64 oracle on 16-frame 1080p / 4K sequences; "
                          "copy: one 1 GiB device copy (counter calibration); composite / pack: the consumer composite / "
                          "the producer pack only; valu: sustained issue rates of FMA, transcendental and mixed "
-                         "instruction chains on this device")
+                         "instruction chains on this device; agents: the agent-frame tick only")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched form (for profiler runs)")
     ap.add_argument("--launch", choices=["persistent", "per_stage"], default="per_stage",
@@ -580,6 +640,9 @@ def main():
     ap.add_argument("--report-threads", type=int, default=16)
     args = ap.parse_args()
 
+    if args.report == "agents":
+        print(json.dumps({"agent_frames": agent_report(args.agents, args.agent_ticks, args.warmup, False)}))
+        return
     if args.report == "valu":
         print(json.dumps({"valu_rates": valu_report()}))
         return

@@ -415,7 +415,11 @@ SVGF_API svgf_status svgf_wire_encode(svgf_wire* wire, const svgf_wire_frame* fr
  * a reader may hold frame n while frames n + 1 and n + 2 are produced. svgf_readback_view gives
  * the host address of a held sequence number: wait = 1 blocks until its copy is done, wait = 0
  * returns SVGF_ERR_NOT_READY while it is in flight; an overwritten or unknown sequence number is
- * SVGF_ERR_INVALID_ARGUMENT. */
+ * SVGF_ERR_INVALID_ARGUMENT. To overlap a push with the next frame's kernels, push on a second
+ * stream once its source is complete on the host's side (an event synchronised after the next
+ * frame's denoise is enqueued): the HIP runtime then copies on the DMA engine, whereas a copy
+ * ordered behind another stream's event by hipStreamWaitEvent is run as a copy kernel whose
+ * writes across PCIe slow the memory-bound kernels beside it several times over. */
 typedef struct svgf_readback svgf_readback;
 
 SVGF_API svgf_status svgf_readback_create(int32_t device, uint64_t slot_bytes, uint32_t slots, svgf_readback** out);
