@@ -162,3 +162,89 @@ def test_agent_frames_end_to_end(S, oracle_lib):
             rgb, depth = O.wire_encode(outs[i].float().cpu().numpy(), gbs[i].cpu().numpy())
             assert np.array_equal(host[rgb_off[i]: rgb_off[i] + 3 * w * h].reshape(h, w, 3), rgb), i
             assert np.array_equal(host[dep_off[i]: dep_off[i] + 4 * w * h].view(np.float32).reshape(h, w), depth), i
+
+
+def test_pipelined_readback_equals_sequential(S):
+    """The overlapped agent tick: tick k encodes into wire buffer k % 2; the host waits for that
+    encode once tick k + 1's denoise is enqueued and pushes the buffer on a second stream; tick
+    k + 2's encode waits (on the device) for that push. Every tick's ring contents equal those
+    of the same ticks run in sequence on one stream, and consecutive ticks differ (a stale or
+    overwritten buffer would show)."""
+    dev = torch.device("cuda")
+    sizes = [(8, 8), (32, 32), (64, 64), (128, 128), (256, 256), (17, 9)]
+    npx = sum(w * h for w, h in sizes)
+    rgb_off, dep_off, off = [], [], 0
+    for w, h in sizes:
+        rgb_off.append(off)
+        off += (3 * w * h + 15) // 16 * 16
+    for w, h in sizes:
+        dep_off.append(off)
+        off += 4 * w * h
+    ticks = 7
+    inputs = []
+    for f in range(2):
+        fr = []
+        for i, (w, h) in enumerate(sizes):
+            d = S.synth_frame(w, h, seed=90 * i + f, stream=i, frame=f, pan_px=0.5)
+            fr.append({"radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                       "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
+                       "motion": torch.from_numpy(d["motion"] * (1.0 if f else -1.0)).to(dev)})
+        inputs.append(fr)
+
+    def run(pipelined):
+        main = torch.cuda.Stream(device=dev)
+        copy = torch.cuda.Stream(device=dev)
+        bufs = [torch.zeros(off, dtype=torch.uint8, device=dev) for _ in range(2)]
+        outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes]
+        enc = [torch.cuda.Event() for _ in range(2)]
+        pushed = [torch.cuda.Event() for _ in range(2)]
+        got = {}
+        with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=256, max_height=256, max_streams=len(sizes),
+                       max_batch_frames=len(sizes), max_batch_pixels=npx) as ctx, \
+                S.Wire(max_frames=len(sizes)) as wire, S.Readback(off, slots=3) as rb:
+            sids = [ctx.stream_open(w, h) for w, h in sizes]
+            torch.cuda.synchronize()
+
+            def frames(k, b):
+                return [{"colour": outs[i], "gbuffer": inputs[k % 2][i]["gbuffer"],
+                         "rgb": bufs[b][rgb_off[i]: rgb_off[i] + 3 * w * h],
+                         "depth": bufs[b][dep_off[i]: dep_off[i] + 4 * w * h].view(torch.float32)}
+                        for i, (w, h) in enumerate(sizes)]
+
+            def push(k):
+                b = k % 2
+                enc[b].synchronize()
+                got[k] = (rb.push(bufs[b].data_ptr(), off, stream=copy), None)
+                pushed[b].record(copy)
+                if k >= 1:  # the ring holds 3 slots: read tick k - 1 before it can be overwritten
+                    q, _ = got[k - 1]
+                    got[k - 1] = (q, rb.view(q).copy())
+
+            for k in range(ticks):
+                b = k % 2 if pipelined else 0
+                items = [{"stream_id": sids[i], "denoised": outs[i], **inputs[k % 2][i]} for i in range(len(sizes))]
+                ctx.denoise_batch(items, stream=main)
+                if pipelined:
+                    if k >= 1:
+                        push(k - 1)
+                    if k >= 2:
+                        main.wait_event(pushed[b])
+                    wire.encode(frames(k, b), stream=main)
+                    enc[b].record(main)
+                else:
+                    wire.encode(frames(k, b), stream=main)
+                    q = rb.push(bufs[b].data_ptr(), off, stream=main)
+                    got[k] = (q, rb.view(q).copy())
+            if pipelined:
+                push(ticks - 1)
+                q, _ = got[ticks - 1]
+                got[ticks - 1] = (q, rb.view(q).copy())
+            torch.cuda.synchronize()
+        return [got[k][1] for k in range(ticks)]
+
+    seq = run(False)
+    pipe = run(True)
+    for k in range(ticks):
+        assert np.array_equal(seq[k], pipe[k]), k
+        if k:
+            assert not np.array_equal(seq[k][: 7 * npx], seq[k - 1][: 7 * npx]), k
