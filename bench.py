@@ -385,6 +385,46 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
     p1.synchronize()
     t_pipe = p0.elapsed_time(p1) / ticks
     t_pipe_den = sum(e0.elapsed_time(e1) for e0, e1 in d_ev) / ticks
+    # Overlapped ticks: as above, and tick k's composite and encode run on a third stream while
+    # tick k + 1's denoise runs. The denoised planes alternate with the tick parity; tick k + 2's
+    # denoise waits for tick k's composite, the last reader of its planes.
+    outs2 = [outs, [torch.empty_like(o) for o in outs]]
+    reqs_o, carr_o = [], []
+    for f in range(2):
+        arr = (S.Request * agents)()
+        ctypes.memmove(arr, reqs[f], ctypes.sizeof(arr))
+        for i in range(agents):
+            arr[i].denoised = S.plane_of(outs2[f][i], S.FMT_RGBA16F)
+        reqs_o.append(arr)
+        carr_o.append(S.Composite.frames([dict(citems[i], history=outs2[f][i]) for i in range(agents)]))
+    sb = torch.cuda.Stream(device=dev)
+    sbh = ctypes.c_void_p(sb.cuda_stream)
+    e_den = [torch.cuda.Event() for _ in range(2)]
+    e_cmp = [torch.cuda.Event() for _ in range(2)]
+    o0, o1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    for k in range(n):
+        b = k % 2  # the previous loop ended on parity 1
+        if k == warmup:
+            o0.record(stream)
+        if k >= 2:
+            stream.wait_event(e_cmp[b])
+        S.check(L.svgf_denoise_batch(ctx.handle, reqs_o[b], agents, sh), "svgf_denoise_batch")
+        e_den[b].record(stream)
+        sb.wait_event(e_den[b])
+        S.check(L.svgf_composite_run(comp.handle, carr_o[b], agents, ctypes.byref(cp), sbh), "svgf_composite_run")
+        e_cmp[b].record(sb)
+        if k >= 1:
+            push(1 - b)
+        if k >= 2:
+            sb.wait_event(e_rb[b])
+        S.check(L.svgf_wire_encode(wire.handle, wfp[b], agents, ctypes.byref(wp), sbh), "svgf_wire_encode")
+        e_enc[b].record(sb)
+    push((n - 1) % 2)
+    stream.wait_event(e_rb[(n - 1) % 2])
+    o1.record(stream)
+    o1.synchronize()
+    t_over = o0.elapsed_time(o1) / ticks
     out = {
         "what": f"{agents} agent streams, eye sizes uniform over 8x8..256x256 (docs/AI_INTERFACE.md:354-360): "
                 "batched FP16 canonical SVGF, consumer composite over the denoised output in one launch, wire encode (fitted ACES + sRGB RGB888, depth in [0, 1]) in one launch, "
@@ -403,6 +443,9 @@ def agent_report(agents: int, ticks: int, warmup: int, cpu: bool) -> dict:
         "readback_GBps": round(off / t_rb / 1e6, 2),
         "pipelined_tick_ms": round(t_pipe, 4),
         "pipelined_denoise_ms": round(t_pipe_den, 4),
+        "overlapped_tick_ms": round(t_over, 4),
+        "overlapped_what": "as pipelined, and each tick's composite and encode on a third stream beside the next tick's "
+                           "denoise (denoised planes alternate with the tick parity)",
         "pipelined_what": "ticks back to back: wire buffers alternate, each tick's readback on a second stream "
                           "(DMA engine) under the next tick's denoise and composite (first timed tick's start to the "
                           "last push's end / ticks)",

@@ -164,14 +164,20 @@ def test_agent_frames_end_to_end(S, oracle_lib):
             assert np.array_equal(host[dep_off[i]: dep_off[i] + 4 * w * h].view(np.float32).reshape(h, w), depth), i
 
 
-def test_pipelined_readback_equals_sequential(S):
-    """The overlapped agent tick: tick k encodes into wire buffer k % 2; the host waits for that
-    encode once tick k + 1's denoise is enqueued and pushes the buffer on a second stream; tick
-    k + 2's encode waits (on the device) for that push. Every tick's ring contents equal those
-    of the same ticks run in sequence on one stream, and consecutive ticks differ (a stale or
-    overwritten buffer would show)."""
+def test_pipelined_ticks_equal_sequential(S):
+    """The overlapped agent tick (denoise -> composite -> encode -> readback), three schedules of
+    the same 7 ticks: in sequence on one stream; pipelined (tick k encodes into wire buffer
+    k % 2, the host waits for that encode once tick k + 1's denoise is enqueued and pushes the
+    buffer on a copy stream, tick k + 2's encode waits on the device for that push); and
+    overlapped (as pipelined, with tick k's composite and encode on a third stream beside tick
+    k + 1's denoise, the denoised planes alternating by parity and tick k + 2's denoise waiting
+    for tick k's composite). Every tick's ring contents are equal across the schedules, and
+    consecutive ticks differ (a stale or overwritten buffer would show)."""
+    from svgf_amd.scenes import composite_scene
+
     dev = torch.device("cuda")
     sizes = [(8, 8), (32, 32), (64, 64), (128, 128), (256, 256), (17, 9)]
+    na = len(sizes)
     npx = sum(w * h for w, h in sizes)
     rgb_off, dep_off, off = [], [], 0
     for w, h in sizes:
@@ -190,61 +196,77 @@ def test_pipelined_readback_equals_sequential(S):
                        "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev),
                        "motion": torch.from_numpy(d["motion"] * (1.0 if f else -1.0)).to(dev)})
         inputs.append(fr)
+    scenes = []
+    for i, (w, h) in enumerate(sizes):
+        d = composite_scene(w, h, 700 + i, half=True)
+        it = {"prev_view_projection": d["m"]}
+        for k in ("emissive", "fresnel", "reflection", "reflection_colour", "reflection_base"):
+            t = torch.from_numpy(d[k]).to(dev)
+            it[k] = t if k == "reflection" else t.half()
+        scenes.append(it)
 
-    def run(pipelined):
-        main = torch.cuda.Stream(device=dev)
-        copy = torch.cuda.Stream(device=dev)
+    def run(mode):
+        main, side, copy = (torch.cuda.Stream(device=dev) for _ in range(3))
+        if mode != "over":
+            side = main
         bufs = [torch.zeros(off, dtype=torch.uint8, device=dev) for _ in range(2)]
-        outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes]
-        enc = [torch.cuda.Event() for _ in range(2)]
-        pushed = [torch.cuda.Event() for _ in range(2)]
+        outs = [[torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes] for _ in range(2)]
+        comp_out = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes]
+        ev = {name: [torch.cuda.Event() for _ in range(2)] for name in ("den", "cmp", "enc", "rb")}
         got = {}
-        with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=256, max_height=256, max_streams=len(sizes),
-                       max_batch_frames=len(sizes), max_batch_pixels=npx) as ctx, \
-                S.Wire(max_frames=len(sizes)) as wire, S.Readback(off, slots=3) as rb:
+        with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=256, max_height=256, max_streams=na,
+                       max_batch_frames=na, max_batch_pixels=npx) as ctx, \
+                S.Composite(max_frames=na) as comp, S.Wire(max_frames=na) as wire, \
+                S.Readback(off, slots=3) as rb:
             sids = [ctx.stream_open(w, h) for w, h in sizes]
             torch.cuda.synchronize()
 
-            def frames(k, b):
-                return [{"colour": outs[i], "gbuffer": inputs[k % 2][i]["gbuffer"],
-                         "rgb": bufs[b][rgb_off[i]: rgb_off[i] + 3 * w * h],
-                         "depth": bufs[b][dep_off[i]: dep_off[i] + 4 * w * h].view(torch.float32)}
-                        for i, (w, h) in enumerate(sizes)]
-
             def push(k):
                 b = k % 2
-                enc[b].synchronize()
+                ev["enc"][b].synchronize()
                 got[k] = (rb.push(bufs[b].data_ptr(), off, stream=copy), None)
-                pushed[b].record(copy)
+                ev["rb"][b].record(copy)
                 if k >= 1:  # the ring holds 3 slots: read tick k - 1 before it can be overwritten
                     q, _ = got[k - 1]
                     got[k - 1] = (q, rb.view(q).copy())
 
             for k in range(ticks):
-                b = k % 2 if pipelined else 0
-                items = [{"stream_id": sids[i], "denoised": outs[i], **inputs[k % 2][i]} for i in range(len(sizes))]
-                ctx.denoise_batch(items, stream=main)
-                if pipelined:
-                    if k >= 1:
-                        push(k - 1)
-                    if k >= 2:
-                        main.wait_event(pushed[b])
-                    wire.encode(frames(k, b), stream=main)
-                    enc[b].record(main)
-                else:
-                    wire.encode(frames(k, b), stream=main)
+                b = k % 2 if mode != "seq" else 0
+                ob = k % 2 if mode == "over" else 0
+                if mode == "over" and k >= 2:
+                    main.wait_event(ev["cmp"][ob])
+                ctx.denoise_batch([{"stream_id": sids[i], "denoised": outs[ob][i], **inputs[k % 2][i]}
+                                   for i in range(na)], stream=main)
+                ev["den"][ob].record(main)
+                side.wait_event(ev["den"][ob])
+                comp.run([dict(scenes[i], history=outs[ob][i], out=comp_out[i]) for i in range(na)], stream=side)
+                ev["cmp"][ob].record(side)
+                wf = [{"colour": comp_out[i], "gbuffer": inputs[k % 2][i]["gbuffer"],
+                       "rgb": bufs[b][rgb_off[i]: rgb_off[i] + 3 * w * h],
+                       "depth": bufs[b][dep_off[i]: dep_off[i] + 4 * w * h].view(torch.float32)}
+                      for i, (w, h) in enumerate(sizes)]
+                if mode == "seq":
+                    wire.encode(wf, stream=main)
                     q = rb.push(bufs[b].data_ptr(), off, stream=main)
                     got[k] = (q, rb.view(q).copy())
-            if pipelined:
+                    continue
+                if k >= 1:
+                    push(k - 1)
+                if k >= 2:
+                    side.wait_event(ev["rb"][b])
+                wire.encode(wf, stream=side)
+                ev["enc"][b].record(side)
+            if mode != "seq":
                 push(ticks - 1)
                 q, _ = got[ticks - 1]
                 got[ticks - 1] = (q, rb.view(q).copy())
             torch.cuda.synchronize()
         return [got[k][1] for k in range(ticks)]
 
-    seq = run(False)
-    pipe = run(True)
-    for k in range(ticks):
-        assert np.array_equal(seq[k], pipe[k]), k
-        if k:
-            assert not np.array_equal(seq[k][: 7 * npx], seq[k - 1][: 7 * npx]), k
+    seq = run("seq")
+    for mode in ("pipe", "over"):
+        other = run(mode)
+        for k in range(ticks):
+            assert np.array_equal(seq[k], other[k]), (mode, k)
+    for k in range(1, ticks):
+        assert not np.array_equal(seq[k][: 7 * npx], seq[k - 1][: 7 * npx]), k
