@@ -258,8 +258,10 @@ SVGF_API svgf_status svgf_budget_used(svgf_context* ctx, uint64_t* bytes);
  * hip_stream, this call's hip_stream waits for it on the device (hipStreamWaitEvent), so the
  * frames of a stream run in call order whatever streams the calls use. SVGF_ERR_CAPACITY: more requests or pixels than svgf_desc allows,
  * or (without SVGF_BATCH_PERSISTENT) a stage with more than 16777215 tiles, which only strides
- * of 256 and 512 reach: a CANONICAL frame has at least 32768 tiles at 9 iterations and 131072
- * at 10 (REFERENCE: twice as many), so keep such batches under 512 and 128 frames (256 and 64). */
+ * of 256 and 512 reach. A REFERENCE frame has at least 65536 tiles at 9 passes and 262144 at 10,
+ * so keep such batches under 256 and 64 frames. A CANONICAL frame whose lattice at the stride has
+ * at most 40 points has one tile per 256 pixels; a wider one has at least s^2 / 2 (32768 at
+ * stride 256, 131072 at 512). */
 SVGF_API svgf_status svgf_denoise_batch(svgf_context* ctx, const svgf_request* requests, uint32_t n,
                                void* hip_stream);
 /* Number of the batch the last successful svgf_denoise_batch call enqueued (0: none yet). */

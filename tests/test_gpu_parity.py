@@ -734,32 +734,51 @@ def test_batch_outcomes_outlive_their_slots(S):
 
 def test_batch_stage_tile_limit(S):
     """Without the persistent form each stage is one launch of one block per tile, and a launch
-    holds at most 2^32 - 1 work-items (16777215 blocks of 256 threads). At 10 iterations a
-    CANONICAL frame has 131072 tiles at stride 512 whatever its size: a batch of 128 frames is
-    refused with SVGF_ERR_CAPACITY before anything is enqueued (no batch number, every stream
-    still at frame 0), one of 127 runs, and each stream's output equals the single-frame
-    form's bit for bit."""
+    holds at most 2^32 - 1 work-items (16777215 blocks of 256 threads). At 10 passes a REFERENCE
+    frame has 262144 single-residue tiles at stride 512 whatever its size: a batch of 64 frames
+    is refused with SVGF_ERR_CAPACITY before anything is enqueued (no batch number, every
+    stream still at frame 0), one of 63 runs, and each stream's output equals the single-frame
+    form's bit for bit. The same 64 frames in CANONICAL mode run in one batch: a lattice of at
+    most 40 points runs direct tiles (one per 256 pixels), so a tiny frame has one tile at every
+    stride."""
     dev = torch.device("cuda")
-    p = S.default_params(S.MODE_CANONICAL, S.STORAGE_FP16)
-    p.iterations = 10
-    n, w, h = 128, 2, 1
+    n, w, h = 64, 2, 1
     frames = [S.synth_frame(w, h, seed=300 + i, frame=0) for i in range(n)]
-    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p, max_width=w, max_height=h, max_streams=n,
-                   max_batch_frames=n, max_batch_pixels=n * w * h) as ctx:
-        sids = [ctx.stream_open(w, h) for _ in range(n)]
-        items = [{"stream_id": sid, "radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
-                  "gbuffer": torch.from_numpy(d["gbuffer"]).to(dev), "motion": torch.from_numpy(d["motion"]).to(dev),
-                  "denoised": torch.empty((h, w, 4), dtype=torch.float16, device=dev)} for sid, d in zip(sids, frames)]
-        with pytest.raises(S.SvgfError) as e:
-            ctx.denoise_batch(items)
-        assert e.value.status == 6  # SVGF_ERR_CAPACITY
-        assert ctx.batch_last() == 0
-        assert all(ctx.stream_state(sid).frames == 0 for sid in sids)
-        ctx.denoise_batch(items[:n - 1])
-        assert ctx.batch_status(ctx.batch_last(), wait=True) == 0
-        assert all(ctx.stream_state(sid).frames == (1 if i < n - 1 else 0) for i, sid in enumerate(sids))
-        torch.cuda.synchronize()
-    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, params=p, max_width=w, max_height=h) as sctx:
-        for i in (0, 1, 63, n - 2):
-            alone = gpu_canonical_sequence(sctx, [frames[i]], torch.float16)
-            assert np.array_equal(alone[0]["denoised"], items[i]["denoised"].float().cpu().numpy()), i
+    for mode in (S.MODE_REFERENCE, S.MODE_CANONICAL):
+        p = S.default_params(mode, S.STORAGE_FP16)
+        p.iterations = 10
+        canonical = mode == S.MODE_CANONICAL
+        with S.Context(mode, S.STORAGE_FP16, params=p, max_width=w, max_height=h, max_streams=n,
+                       max_batch_frames=n, max_batch_pixels=n * w * h) as ctx:
+            sids = [ctx.stream_open(w, h) for _ in range(n)]
+            items = []
+            for sid, d in zip(sids, frames):
+                it = {"stream_id": sid, "radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                      "denoised": torch.empty((h, w, 4), dtype=torch.float16, device=dev)}
+                if canonical:
+                    it["gbuffer"] = torch.from_numpy(d["gbuffer"]).to(dev)
+                    it["motion"] = torch.from_numpy(d["motion"]).to(dev)
+                items.append(it)
+            run = items
+            if not canonical:
+                with pytest.raises(S.SvgfError) as e:
+                    ctx.denoise_batch(items)
+                assert e.value.status == 6  # SVGF_ERR_CAPACITY
+                assert ctx.batch_last() == 0
+                assert all(ctx.stream_state(sid).frames == 0 for sid in sids)
+                run = items[:n - 1]
+            ctx.denoise_batch(run)
+            assert ctx.batch_status(ctx.batch_last(), wait=True) == 0
+            assert all(ctx.stream_state(sid).frames == (1 if i < len(run) else 0) for i, sid in enumerate(sids))
+            torch.cuda.synchronize()
+        with S.Context(mode, S.STORAGE_FP16, params=p, max_width=w, max_height=h) as sctx:
+            for i in (0, 1, 31, len(run) - 1):
+                if canonical:
+                    alone = gpu_canonical_sequence(sctx, [frames[i]], torch.float16)
+                    assert np.array_equal(alone[0]["denoised"], items[i]["denoised"].float().cpu().numpy()), i
+                else:
+                    mom = torch.zeros((h, w, 2), dtype=torch.float16, device=dev)
+                    den = torch.empty((h, w, 4), dtype=torch.float16, device=dev)
+                    sctx.denoise(radiance=items[i]["radiance"], denoised=den, moments_curr=mom, history_valid=False)
+                    torch.cuda.synchronize()
+                    assert torch.equal(den, items[i]["denoised"]), i

@@ -721,6 +721,56 @@ struct Sums {
     }
 };
 
+// A-trous iteration k >= 1 (stride s = 2^k) on a direct tile: pixels T.i0 .. T.i0 + 255 of the
+// frame (row-major), one per thread, each tap (and the variance prefilter's 3x3) loaded from
+// memory at clamped coordinates, the values a pair tile stages for the same pixel; the same
+// arithmetic in the same order as the staged filter below, so the outputs are the same floats.
+// Used for frames whose stride-s lattice is tiny (svgf_sched.hpp, direct_lattice), where the
+// lanes of neighbouring pixels load neighbouring taps.
+template <class S, bool LAST>
+__device__ __forceinline__ void can_atrous_direct(const FrameDesc& f, const KParams& p, const Tile& T, int k)
+{
+    const int s = 1 << k;
+    const Buf src = bind((k & 1) ? f.wb : f.wa, f.h), gb = bind(f.g, f.h), gr = bind(f.grad, f.h);
+    const int max_x = f.w - 1, max_y = f.h - 1;
+    const int pix = T.i0 + lane_tid();
+    if (pix < f.w * f.h) {
+        const int y = pix / f.w, x = pix - y * f.w;
+        auto qx = [&](int dx) { return clampi(x + dx * s, 0, max_x); };
+        auto qy = [&](int dy) { return clampi(y + dy * s, 0, max_y); };
+        // 3x3 binomial of the variance over the lattice neighbours
+        float gv = 0.0f;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const float sd = WavS::ld4(src, qx(dx), qy(dy)).w;
+                gv = fmaf((dx == 0 ? 0.5f : 0.25f) * (dy == 0 ? 0.5f : 0.25f), sd * sd, gv);
+            }
+        float4 C = WavS::ld4(src, x, y);
+        const float Vc = C.w * C.w;
+        C.w = luma_fma(C.x, C.y, C.z);
+        const float2 g = ld_f2(gr, y * gr.pitch + x * 8);
+        const Centre c = centre_of(p, C, ldg4(gb, x, y), gv, g.x, g.y, s);
+        Sums<!LAST> a;
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+            for (int dx = -2; dx <= 2; ++dx) {
+                float4 A = WavS::ld4(src, qx(dx), qy(dy));
+                const float V = LAST ? 0.0f : A.w * A.w;
+                A.w = luma_fma(A.x, A.y, A.z);
+                a.add(tap_weight(p, c, dx, dy, A, ldg4(gb, qx(dx), qy(dy))), A, V);
+            }
+            a.pin();
+        }
+        const float4 o = a.result(C, Vc);
+        if (LAST || k == p.iterations - 1) S::st4(bind(f.den, f.h), x, y, make_float4(o.x, o.y, o.z, 1.0f));
+        else WavS::st4(bind((k & 1) ? f.wa : f.wb, f.h), x, y, o);
+    }
+    __syncthreads();  // the tile functions of the persistent kernel's ticket loop all end at a barrier
+}
+
 // A-trous iteration k (stride s = 2^k) on one lattice tile: luminance, depth and normal
 // edge-stops, variance filtered by squared weights. Iteration 0 also writes the colour
 // history, the last one the output.
@@ -754,6 +804,12 @@ __device__ __forceinline__ void can_atrous_tile(const FrameDesc& f, const KParam
     constexpr int QW = LT + 2 * LHALO, QH = TH + 2 * LHALO;  // tap region
     constexpr uint32_t NA = NR * RR, LIST = FIRST ? QW * QH : 0;
     static_assert(((NA * 36 + LIST * 2 + 15) & ~15u) + 16 == atrous_lds_bytes(FIRST, NR, TALL), "LDS layout");
+    if constexpr (NR == 2) {
+        if (T.rx < 0) {  // a direct tile (uniform across the block)
+            can_atrous_direct<S, LAST>(f, p, T, k);
+            return;
+        }
+    }
     char* const lds = L::template get<atrous_lds_bytes(FIRST, NR, TALL)>();
     float4* const sA = reinterpret_cast<float4*>(lds);                // (r, g, b, luminance), residue-major
     float4* const sB = reinterpret_cast<float4*>(lds + NA * 16);      // (normal, depth)
@@ -1009,9 +1065,9 @@ __device__ __forceinline__ FrameDesc load_frame(const FrameDesc* frames, uint32_
 
 // The tile of stage STAGE that logical workgroup t of a frame runs.
 template <int STAGE>
-__device__ __forceinline__ Tile stage_tile(int w, uint32_t log2s, uint32_t t)
+__device__ __forceinline__ Tile stage_tile(int w, int h, uint32_t log2s, uint32_t t)
 {
-    if constexpr (is_pair_stage(STAGE)) return decode_pair_tile(w, log2s, t);
+    if constexpr (is_pair_stage(STAGE)) return decode_pair_tile(w, h, log2s, t);
     else if constexpr (STAGE == ST_CAN_ATROUS0) return decode_tall_tile(w, t);
     else return decode_tile(w, log2s, t);
 }
@@ -1020,7 +1076,7 @@ __device__ __forceinline__ Tile stage_tile(int w, uint32_t log2s, uint32_t t)
 template <class S, int STAGE, int SF>
 __global__ __launch_bounds__(BLOCK) void k_frame(FrameDesc f, KParams p, int arg, uint32_t log2s)
 {
-    Tile T = stage_tile<STAGE>(f.w, log2s, xcd_logical(blockIdx.x, gridDim.x));
+    Tile T = stage_tile<STAGE>(f.w, f.h, log2s, xcd_logical(blockIdx.x, gridDim.x));
     run_stage<S, STAGE, SF>(f, p, T, arg);
 }
 
@@ -1035,7 +1091,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchTable bt, KParams p, int a
     if (t >= starts[nf]) return;
     const uint32_t fi = frame_of(starts, nf, t);
     const FrameDesc f = load_frame(bt.frames, fi);
-    Tile T = stage_tile<STAGE>(f.w, log2s, t - starts[fi]);
+    Tile T = stage_tile<STAGE>(f.w, f.h, log2s, t - starts[fi]);
     run_stage<S, STAGE, SF>(f, p, T, arg);
 }
 
@@ -1048,7 +1104,7 @@ __device__ __forceinline__ void persist_tile(const FrameDesc* frames, uint32_t f
 {
     const FrameDesc f = load_frame(frames, fi);
     constexpr bool CANON = STAGE != ST_REF_INIT && STAGE != ST_REF_ATROUS;
-    run_stage<S, STAGE, SF, ArenaLds<CANON>>(f, p, stage_tile<STAGE>(f.w, log2s, tile), arg);
+    run_stage<S, STAGE, SF, ArenaLds<CANON>>(f, p, stage_tile<STAGE>(f.w, f.h, log2s, tile), arg);
 }
 
 // Persistent form: every stage of a batch in one launch. Each workgroup takes chunks of

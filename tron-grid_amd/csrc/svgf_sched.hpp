@@ -53,8 +53,22 @@ SVGF_HD inline bool paired_stage(int canonical, int stage, uint32_t log2s)
     return canonical && stage >= 2 && log2s >= PAIR_LOG2S;
 }
 
+// A frame whose stride-s lattice has at most DIRECT_MAX_POINTS points (ceil(w/s) ceil(h/s): an
+// 8x8 agent eye from stride 2 on, a 64x64 one at stride 16) would leave most of every pair tile's
+// 2 x 16 x 16 slots idle, each tile staging its whole region for a few points. Such a frame
+// runs direct tiles instead: 256 consecutive pixels (row-major) per tile, one per thread, every
+// tap loaded from memory (adjacent lanes' taps are adjacent pixels), ceil(w h / 256) tiles.
+constexpr uint32_t DIRECT_MAX_POINTS = 40;
+
+SVGF_HD inline bool direct_lattice(uint32_t w, uint32_t h, uint32_t log2s)
+{
+    const uint32_t s = 1u << log2s;
+    return ((w + s - 1) >> log2s) * ((h + s - 1) >> log2s) <= DIRECT_MAX_POINTS;
+}
+
 SVGF_HD inline uint32_t pair_tiles_at(uint32_t w, uint32_t h, uint32_t log2s)
 {
+    if (direct_lattice(w, h, log2s)) return (w * h + BLOCK - 1) / BLOCK;
     return tiles_at(w, h, log2s) / 2;
 }
 
@@ -94,9 +108,11 @@ SVGF_HD inline Tile decode_tile(int w, uint32_t log2s, uint32_t t)
 }
 
 // Tile t of a residue-pair stage (stride s >= 2): residues (rx, ry) and (rx + 1, ry), rx even,
-// ordered as decode_tile orders single residues.
-SVGF_HD inline Tile decode_pair_tile(int w, uint32_t log2s, uint32_t t)
+// ordered as decode_tile orders single residues; or, for a frame of direct tiles, rx = -1 and
+// i0 = its first pixel (row-major index).
+SVGF_HD inline Tile decode_pair_tile(int w, int h, uint32_t log2s, uint32_t t)
 {
+    if (direct_lattice((uint32_t)w, (uint32_t)h, log2s)) return Tile{(int)log2s, -1, 0, (int)(t * BLOCK), 0};
     const uint32_t s = 1u << log2s;
     const uint32_t lx = ((uint32_t)w + s - 1) >> log2s;
     const uint32_t ntx = (lx + LT - 1) / LT;
