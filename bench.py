@@ -103,8 +103,9 @@ def valu_calibration():
     return {"fma_wave_instr_per_s": round(r["fma"], -6), "exp_1_in_8_wave_instr_per_s": round(r["exp2_fma_1_7"], -6),
             "fma_cycles_per_instr_at_2p4GHz": round(4 * nominal / r["fma"], 3),
             "what": "best of 5 launches of 16 independent v_fma_f32 chains per thread, 8 waves per SIMD on "
-                    "every CU (and the same with one chain in eight on v_exp_f32): the issue rate pure FMA "
-                    "work sustains on this device under load (the clock drops below 2.4 GHz)"}
+                    "every CU (and the same with one chain in eight on v_exp_f32): the issue rate chains of "
+                    "three-source FMAs sustain on this device, a floor of the VALU issue rate (the a-trous kernels, "
+                    "mostly two-source FMAs, issue every 4 cycles at the 2.1-2.2 GHz they run at)"}
 
 
 def valu_report() -> dict:
@@ -1251,7 +1252,7 @@ def main():
                          "achieved_wave_instr_per_s": round(valu_insts / (dom_ms * 1e-3), -6),
                          "issue_frac": round(valu_insts / (dom_ms * 1e-3) / valu_issue_per_s, 4),
                          "peak": "one wave64 VALU instruction per SIMD per 4 cycles, 1024 SIMDs, 2.4 GHz "
-                                 "(a lower clock under load raises the fraction)",
+                                 "(the clock under the a-trous load is 2.1-2.2 GHz: a lower clock raises the fraction)",
                          "sustained": calib,
                          "issue_frac_sustained": None if calib is None else round(
                              valu_insts / (dom_ms * 1e-3) / calib["fma_wave_instr_per_s"], 4),
