@@ -9,13 +9,16 @@ Arrays are numpy, tightly packed, row-major, channels interleaved: colour-like p
 from __future__ import annotations
 
 import ctypes
+import os
 import pathlib
 import subprocess
 
 import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
-LIB_PATH = HERE / "_build" / "libsvgf_oracle.so"
+# SVGF_ORACLE_LIB names another build of the same sources (the sanitizer test's)
+LIB_PATH = pathlib.Path(os.environ["SVGF_ORACLE_LIB"]) if "SVGF_ORACLE_LIB" in os.environ else \
+    HERE / "_build" / "libsvgf_oracle.so"
 
 FLAG_COMPENSATED = 0x1
 FLAG_AS_REFERENCE = 0x100

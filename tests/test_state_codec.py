@@ -33,6 +33,19 @@ def test_codec_bounds(codec_dump):
     assert len(codec_dump) == 2_000_000
 
 
+def test_codec_under_address_and_ub_sanitizers(tmp_path):
+    """The same codec program with AddressSanitizer and UndefinedBehaviorSanitizer (shifts of the
+    exponent and residual codes, conversions, out-of-bounds): it still meets its bounds."""
+    if not CLANG.exists():
+        pytest.skip("ROCm clang++ not available")
+    exe = tmp_path / "state_codec_san"
+    subprocess.run([str(CLANG), "-std=c++20", "-O1", "-g", "-Wall", "-Wextra", "-Werror",
+                    "-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-o", str(exe),
+                    str(HERE / "state_codec.cpp")], check=True)
+    out = subprocess.run([str(exe), str(tmp_path / "dump.bin")], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+
+
 def test_numpy_decoder_matches_codec(codec_dump):
     v, h, code = codec_dump[:, 0], codec_dump[:, 1].astype(np.float16), codec_dump[:, 2].astype(np.int32)
     assert np.array_equal(h.astype(np.float32), codec_dump[:, 1])  # every h is a binary16 value
