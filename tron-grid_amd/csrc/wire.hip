@@ -72,14 +72,19 @@ __device__ __forceinline__ void mat3(const float* m, const float v[3], float o[3
     for (int r = 0; r < 3; ++r) o[r] = __builtin_fmaf(m[3 * r], v[0], __builtin_fmaf(m[3 * r + 1], v[1], m[3 * r + 2] * v[2]));
 }
 
-// count of thresholds <= x (branch-free search over the 255 ascending thresholds in LDS)
+// Code of x in [0, 1]: the count of thresholds <= x, with t[k] the k-th threshold (k = 1..255)
+// and t[0] = -inf, t[256] = +inf in LDS. The sRGB curve in float (hardware log2 / exp2) puts the
+// estimate e = floor(255 srgb(x) + 1/2) within one code of the count: its error, ~1e-5 of the
+// curve's range, is far below one code step (1/255). One comparison on each side of e then gives
+// the count exactly: two independent LDS reads instead of a dependent 8-step search.
 __device__ __forceinline__ uint32_t code_of(const float* t, float x)
 {
-    uint32_t c = 0;
-#pragma unroll
-    for (uint32_t s = 128; s > 0; s >>= 1)
-        if (c + s <= 255u && t[c + s - 1] <= x) c += s;
-    return c;
+    const float s = x <= 0.0031308f ? x * 12.92f
+                                    : __builtin_fmaf(1.055f, __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 2.4f)),
+                                                     -0.055f);
+    const int e = min(max((int)__builtin_fmaf(s, 255.0f, 0.5f), 0), 255);
+    const float below = t[e], above = t[e + 1];  // thresholds e and e + 1
+    return (uint32_t)(above <= x ? e + 1 : (below > x ? e - 1 : e));
 }
 
 __device__ __forceinline__ uint32_t encode_rgb(const float* t, float r, float g, float b, float exposure)
@@ -120,9 +125,13 @@ __device__ __forceinline__ float encode_depth(float z, float near_z, float far_z
 template <bool FP16>
 __global__ __launch_bounds__(WIRE_BLOCK) void k_wire(WireArgs a)
 {
-    __shared__ float sT[256];
+    __shared__ float sT[257];
     const int tid = (int)threadIdx.x;
-    if (tid < 255) sT[tid] = a.thresholds[tid];
+    if (tid < 255) sT[tid + 1] = a.thresholds[tid];
+    if (tid == 255) {
+        sT[0] = -__builtin_inff();
+        sT[256] = __builtin_inff();
+    }
     const uint32_t b = blockIdx.x;
     if (b >= a.starts[a.n]) return;  // uniform per block (before any barrier)
     __syncthreads();
