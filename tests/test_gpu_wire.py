@@ -270,3 +270,77 @@ def test_pipelined_ticks_equal_sequential(S):
             assert np.array_equal(seq[k], other[k]), (mode, k)
     for k in range(1, ticks):
         assert not np.array_equal(seq[k][: 7 * npx], seq[k - 1][: 7 * npx]), k
+
+
+def test_agent_tick_at_256_streams(S, oracle_lib):
+    """One agent tick at the bench's size: 256 resident streams at eye sizes drawn from
+    docs/AI_INTERFACE.md:354-360 (8x8 .. 256x256, 4.6 Mpx), a second frame of each stream denoised
+    in one batch, the composite over the denoised output in one launch, the wire encode in one
+    launch, and one push into the ring. For 16 sampled agents (every size class) the composite
+    equals the composite oracle on the GPU's denoised output and the ring's bytes equal the wire
+    oracle's encoding of the GPU's composited output, bit for bit."""
+    from svgf_amd.scenes import composite_scene
+
+    O = oracle_lib
+    dev = torch.device("cuda")
+    agent_sizes = [(8, 8), (32, 32), (64, 64), (128, 128), (256, 256)]
+    rng = np.random.default_rng(2)
+    n = 256
+    sizes = [agent_sizes[i] for i in rng.integers(0, len(agent_sizes), n)]
+    npx = sum(w * h for w, h in sizes)
+    rgb_off, dep_off, off = [], [], 0
+    for w, h in sizes:
+        rgb_off.append(off)
+        off += (3 * w * h + 15) // 16 * 16
+    for w, h in sizes:
+        dep_off.append(off)
+        off += 4 * w * h
+    packed = torch.zeros(off, dtype=torch.uint8, device=dev)
+    cplanes = ("emissive", "fresnel", "reflection", "reflection_colour", "reflection_base")
+    with S.Context(S.MODE_CANONICAL, S.STORAGE_FP16, max_width=256, max_height=256, max_streams=n,
+                   max_batch_frames=n, max_batch_pixels=npx) as ctx, \
+            S.Composite(max_frames=n) as comp, S.Wire(max_frames=n) as wire, S.Readback(off, slots=3) as rb:
+        sids = [ctx.stream_open(w, h) for w, h in sizes]
+        outs = [torch.empty((h, w, 4), dtype=torch.float16, device=dev) for w, h in sizes]
+        gbs = None
+        for f in range(2):
+            items, gbs = [], []
+            for i, (w, h) in enumerate(sizes):
+                d = S.synth_frame(w, h, seed=11 * i + f, stream=i, frame=f, pan_px=0.5, disocclusion=0.05)
+                gb = torch.from_numpy(d["gbuffer"]).to(dev)
+                gbs.append(gb)
+                items.append({"stream_id": sids[i], "radiance": torch.from_numpy(d["radiance"]).to(dev).half(),
+                              "gbuffer": gb, "motion": torch.from_numpy(d["motion"]).to(dev), "denoised": outs[i]})
+            ctx.denoise_batch(items)
+        citems = []
+        for i, (w, h) in enumerate(sizes):
+            d = composite_scene(w, h, 5000 + i, half=True)
+            it = {"prev_view_projection": d["m"], "history": outs[i],
+                  "out": torch.empty((h, w, 4), dtype=torch.float16, device=dev)}
+            for k in cplanes:
+                t = torch.from_numpy(d[k]).to(dev)
+                it[k] = t if k == "reflection" else t.half()
+            citems.append(it)
+        comp.run(citems)
+        wire.encode([{"colour": citems[i]["out"], "gbuffer": gbs[i],
+                      "rgb": packed[rgb_off[i]: rgb_off[i] + 3 * w * h],
+                      "depth": packed[dep_off[i]: dep_off[i] + 4 * w * h].view(torch.float32)}
+                     for i, (w, h) in enumerate(sizes)])
+        host = rb.view(rb.push(packed.data_ptr(), off)).copy()
+        torch.cuda.synchronize()
+        by_class = {}
+        for i, wh in enumerate(sizes):
+            by_class.setdefault(wh, []).append(i)
+        sample = sorted({i for idx in by_class.values() for i in idx[:3]} | {0, n - 1})[:16]
+        assert {sizes[i] for i in sample} == set(agent_sizes)
+        for i in sample:
+            w, h = sizes[i]
+            it = citems[i]
+            want = O.composite(it["prev_view_projection"], *[it[k].float().cpu().numpy() for k in
+                               ("history", "emissive", "fresnel", "reflection", "reflection_colour",
+                                "reflection_base")], store=O.STORE_F16)
+            got = it["out"].float().cpu().numpy()
+            assert np.array_equal(got, want), i
+            rgb, depth = O.wire_encode(got, gbs[i].cpu().numpy())
+            assert np.array_equal(host[rgb_off[i]: rgb_off[i] + 3 * w * h].reshape(h, w, 3), rgb), i
+            assert np.array_equal(host[dep_off[i]: dep_off[i] + 4 * w * h].view(np.float32).reshape(h, w), depth), i
