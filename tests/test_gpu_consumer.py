@@ -125,3 +125,18 @@ def test_compiled_batch_host_vs_oracle(S, oracle_lib, tmp_path, storage):
             assert rel_err(mom, ref[-1]["moments"])[0] <= bound, i
         assert rel_err(h3, ref[-1]["history"][..., :3])[0] <= bound, i
         assert np.array_equal(n, ref[-1]["history"][..., 3]), i
+
+
+def test_compiled_agent_tick_host(S):
+    """tests/consumer/svgf_agent_consumer (C++20, hipcc) runs the agent tick of the integration
+    guide through the C ABI alone: per tick one batched denoise of 15 agent streams (eyes of
+    8x8 .. 256x256), the composite over the denoised output, the wire encode and one push into
+    the pinned ring, in sequence on one stream and pipelined (wire buffers alternating, each push
+    on a copy stream once the host has seen its encode finish, the encode two ticks later
+    waiting on the device for that push). Every tick's ring bytes are equal in both schedules and
+    consecutive ticks differ."""
+    path = BUILD / "svgf_agent_consumer"
+    assert path.exists(), f"{path} is not built (make -C tests/consumer, run by __graft_entry__.build())"
+    r = subprocess.run([str(path), "15", "6"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("ok ")
