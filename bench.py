@@ -658,7 +658,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP64-oracle check of the timed batched output")
     ap.add_argument("--agents", type=int, default=256, help="agent streams of the agent-frame tick (0: skip)")
-    ap.add_argument("--agent-ticks", type=int, default=20)
+    ap.add_argument("--agent-ticks", type=int, default=60)
     ap.add_argument("--composite-reps", type=int, default=50,
                     help="launches timed per consumer-composite and producer-pack case (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
